@@ -1,0 +1,201 @@
+/*
+ * rcgrid.h -- C ABI of the MI355X (gfx950) w-projection gridder/degridder.
+ *
+ * Two layers:
+ *
+ *  1. The reference's MS4 kernel entry points, same names, argument order and
+ *     meaning, all buffers in HOST memory, so that the Haskell DNA layer
+ *     (MS4/dna-programs/Kernel/CPU/ScatterGrid.hs:48-51,87) and the C++
+ *     driver (MS4/misc/cppcycle.cpp:210-292) link against this library
+ *     unchanged.  Each call copies its inputs to the GPU, runs the HIP
+ *     kernels and copies the result back; nothing persists between calls.
+ *     Types: rcg_c128 and rcg_double3 have the layout of the reference's
+ *     complexd (std::complex<double>) and Double3 (MS4/kernel/common/
+ *     common.h:37-39,72-77); every one of them is passed by pointer.
+ *
+ *  2. A device-resident API (rcg_*) on caller-owned device buffers and a
+ *     caller-chosen HIP stream, used by the Python binding, the benchmark
+ *     and the batched/FP32 paths.
+ *
+ * Errors: the MS4 functions have no error path in the reference.  Here a
+ * failing call returns 0 taps (or does nothing) and leaves the output zeroed
+ * where it can; rcg_last_error() / rcg_last_error_string() report the cause
+ * (thread-local).  All rcg_* functions return an rcg_status.
+ */
+#ifndef RCGRID_H
+#define RCGRID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef unsigned long long ull;
+
+typedef struct rcg_c128 { double re, im; } rcg_c128;
+typedef struct rcg_c64 { float re, im; } rcg_c64;
+typedef struct rcg_double3 { double u, v, w; } rcg_double3;
+
+typedef enum rcg_status {
+  RCG_OK = 0,
+  RCG_ERR_ARG = 1,        /* invalid argument (sizes, pointers, supports) */
+  RCG_ERR_HIP = 2,        /* a HIP runtime call failed                     */
+  RCG_ERR_NOMEM = 3,      /* device or host allocation failed              */
+  RCG_ERR_RANGE = 4,      /* a point referenced a w-plane or GCF layer
+                             outside the table, or a GCF window that does
+                             not fit its layer                              */
+  RCG_ERR_UNSUPPORTED = 5,
+  RCG_ERR_FFT = 6         /* rocFFT plan or execution failed               */
+} rcg_status;
+
+/* ---------------------------------------------------------------------
+ * 1. Reference MS4 entry points (host memory)
+ * ------------------------------------------------------------------- */
+
+/* Replaces gridKernelCPUFullGCF / gridKernelCPUHalfGCF,
+ * MS4/kernel/common/scatter_gridder_w_dependent_dyn_1p.h:16-34 (bodies
+ * MS4/kernel/cpu/scatter_grid/scatter_gridder_w_dependent_dyn_1p.cpp:
+ * 245-266).  Zeroes grid (grid_size rows x grid_pitch complex), grids
+ * baselines*ts_ch points; uvw[bl], vis[bl] point at ts_ch consecutive
+ * points of chunk bl whose support is bl_supps[bl].  gcf and gcf_supps point
+ * at the w_plane = 0 centre of the layer-pointer and support tables
+ * (negative indices allowed).  Returns the number of in-bound taps.
+ * Accumulation is FP64 (result within 1e-12 relative L2 of the reference). */
+ull gridKernelCPUFullGCF(double scale, double wstep, int baselines,
+                         const int bl_supps[], rcg_c128 grid[],
+                         const rcg_c128 *gcf[], const rcg_double3 *uvw[],
+                         const rcg_c128 *vis[], int ts_ch, int grid_pitch,
+                         int grid_size, int gcf_supps[]);
+ull gridKernelCPUHalfGCF(double scale, double wstep, int baselines,
+                         const int bl_supps[], rcg_c128 grid[],
+                         const rcg_c128 *gcf[], const rcg_double3 *uvw[],
+                         const rcg_c128 *vis[], int ts_ch, int grid_pitch,
+                         int grid_size, int gcf_supps[]);
+
+/* Replaces deGridKernelCPUFullGCF / deGridKernelCPUHalfGCF,
+ * scatter_gridder_w_dependent_dyn_1p.h:73-91 (bodies .cpp:368-431).
+ * Overwrites every vis[bl][i] with the transpose-degridded value
+ * (unconjugated taps; out-of-bound points become 0).  Returns taps. */
+ull deGridKernelCPUFullGCF(double scale, double wstep, int baselines,
+                           const int bl_supps[], const rcg_c128 grid[],
+                           const rcg_c128 *gcf[], const rcg_double3 *uvw[],
+                           rcg_c128 *vis[], int ts_ch, int grid_pitch,
+                           int grid_size, int gcf_supps[]);
+ull deGridKernelCPUHalfGCF(double scale, double wstep, int baselines,
+                           const int bl_supps[], const rcg_c128 grid[],
+                           const rcg_c128 *gcf[], const rcg_double3 *uvw[],
+                           rcg_c128 *vis[], int ts_ch, int grid_pitch,
+                           int grid_size, int gcf_supps[]);
+
+/* Replaces grid0, scatter_gridder_w_dependent_dyn_1p.h:36-45 (.cpp:268-291):
+ * nearest-cell gridding into a grid pointer the caller has centred. */
+void grid0(const rcg_double3 uvw[], const rcg_c128 vis[], rcg_c128 grid[],
+           double scale, int baselines_ts_ch, int grid_pitch, int grid_size);
+
+/* Replaces normalizeCPU, .h:47-52 (.cpp:293-306): grid *= 1/(size*pitch). */
+void normalizeCPU(rcg_c128 src[], int grid_pitch, int grid_size);
+
+/* Replaces reweight, .h:54-61 (.cpp:311-353): uniform weighting. */
+void reweight(const rcg_double3 uvw[], rcg_c128 vis[], double scale,
+              int baselines_ts_ch, int grid_size);
+
+/* Replaces rotateCPU, .h:63-69 (.cpp:355-363): vis *= exp(2 pi i w scale),
+ * evaluated in FP64. */
+void rotateCPU(const rcg_double3 uvw[], rcg_c128 vis[], int baselines_ts_ch,
+               double scale);
+
+/* Replaces herm_padded_inplace, MS4/kernel/common/herm_padded.h:9
+ * (MS4/kernel/cpu/herm/herm_padded.cpp:6-27). */
+void herm_padded_inplace(rcg_c128 *data, int size, int pitch);
+
+/* Replaces fft_inplace_even / fftInitThreading,
+ * MS4/kernel/common/fft_dyn_padded.h:10-11 (MS4/kernel/cpu/fft/
+ * fft_dyn_padded.cpp:77-127).  sign: -1 forward C2C, +1 backward C2C,
+ * 0 R2C, 2 C2R, with the reference's fftshift before and after and its
+ * in-place strides.  Returns an opaque plan handle (the argument plan may be
+ * NULL); release it with rcg_fft_destroy_plan (the reference's Haskell
+ * binding calls fftw_destroy_plan, MS4/dna-programs/Kernel/CPU/FFT.hs:23). */
+void *fft_inplace_even(void *plan, int sign, void *data, int size, int pitch);
+void fftInitThreading(void);
+void rcg_fft_destroy_plan(void *plan);
+
+/* Replaces deconvolve, MS4/kernel/common/hogbom.h:7-16
+ * (MS4/kernel/cpu/hogbom/hogbom.cpp:94-121) with textbook Hogbom CLEAN:
+ * signed peak, model += gain*peak/psf_peak, stop when |peak| < threshold. */
+void deconvolve(double *mod_p, double *res_p, double *psf_p, int siz,
+                int pitch, unsigned int niters, double gain, double threshold);
+
+/* Thread-local error of the last call made on this thread (0 = none). */
+int rcg_last_error(void);
+const char *rcg_last_error_string(void);
+
+/* ---------------------------------------------------------------------
+ * 2. Device-resident API
+ * ------------------------------------------------------------------- */
+
+/* A GCF table in device memory.  Layer slot s (0 <= s < n_slots) starts at
+ * element layer_off[s] of data and holds layer_len[s] complex taps, row
+ * stride = its plane's support.  The slot of layer index L (the reference's
+ * signed (w_plane*8+over_u)*8+over_v) is L + center for a full table and
+ * |L| for a half table (center must be 0).  supps[p + max_w_plane] is the
+ * support of w-plane p for -max_w_plane <= p <= max_w_plane.
+ * data is rcg_c128 for the FP64 entry points and rcg_c64 for FP32 ones. */
+typedef struct rcg_gcf {
+  const void *data;
+  const int64_t *layer_off;
+  const int32_t *layer_len;
+  int32_t n_slots;
+  int32_t center;
+  const int32_t *supps;
+  int32_t max_w_plane;
+  int32_t half;
+} rcg_gcf;
+
+/* Visibility geometry in device memory: n_chunks chunks of ts_ch points,
+ * uvw[n_chunks*ts_ch] (metres or wavelengths; scaled by `scale`), the
+ * per-chunk support chunk_supps[n_chunks]. */
+typedef struct rcg_geom {
+  const rcg_double3 *uvw;
+  const int32_t *chunk_supps;
+  int32_t n_chunks;
+  int32_t ts_ch;
+  double scale;
+  double wstep;
+  int32_t grid_size;
+  int32_t grid_pitch;
+} rcg_geom;
+
+/* Per-point record written by rcg_pregrid (16 bytes). */
+typedef struct rcg_pg {
+  int32_t cell;     /* u0*grid_pitch + v0 of the window's top-left cell,
+                       or -1 when the point is not in-bound            */
+  int32_t tap0;     /* element offset of the recentred window in data  */
+  int16_t stride;   /* layer row stride (plane support)                */
+  int16_t supp;     /* window support (chunk support)                  */
+  int32_t flags;    /* bit 0: conjugate taps; bit 1: range error       */
+} rcg_pg;
+
+/* FP64 exact pregrid of every point (reference pregridPoint + SG:128-144).
+ * pg must hold n_chunks*ts_ch records.  *taps_dev (device, optional)
+ * receives the in-bound tap count.  Status RCG_ERR_RANGE is reported by
+ * rcg_plain_check() after the stream is synchronised. */
+rcg_status rcg_pregrid(const rcg_geom *geom, const rcg_gcf *gcf, rcg_pg *pg,
+                       unsigned long long *taps_dev, int *err_dev,
+                       void *stream);
+
+/* FP64 "plain" path: one launch over all points, one wavefront per point,
+ * FP64 global atomics.  grid is zeroed first.  vis/grid rcg_c128. */
+rcg_status rcg_grid_plain_f64(const rcg_geom *geom, const rcg_gcf *gcf,
+                              const rcg_pg *pg, const rcg_c128 *vis,
+                              rcg_c128 *grid, void *stream);
+rcg_status rcg_degrid_plain_f64(const rcg_geom *geom, const rcg_gcf *gcf,
+                                const rcg_pg *pg, const rcg_c128 *grid,
+                                rcg_c128 *vis, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCGRID_H */

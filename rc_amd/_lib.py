@@ -1,0 +1,102 @@
+"""ctypes binding of rc_amd/librcgrid.so (C ABI declared in include/rcgrid.h).
+
+The library is the product: nothing here falls back to a CPU or PyTorch
+implementation.  If the shared library is missing or fails to load, every
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "librcgrid.so"
+
+ull = C.c_ulonglong
+c_int_p = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+
+class RcgError(RuntimeError):
+    """A librcgrid call reported an error (rcg_last_error)."""
+
+
+class Geom(C.Structure):
+    """struct rcg_geom"""
+    _fields_ = [
+        ("uvw", vp),
+        ("chunk_supps", vp),
+        ("n_chunks", C.c_int32),
+        ("ts_ch", C.c_int32),
+        ("scale", C.c_double),
+        ("wstep", C.c_double),
+        ("grid_size", C.c_int32),
+        ("grid_pitch", C.c_int32),
+    ]
+
+
+class Gcf(C.Structure):
+    """struct rcg_gcf"""
+    _fields_ = [
+        ("data", vp),
+        ("layer_off", vp),
+        ("layer_len", vp),
+        ("n_slots", C.c_int32),
+        ("center", C.c_int32),
+        ("supps", vp),
+        ("max_w_plane", C.c_int32),
+        ("half", C.c_int32),
+    ]
+
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+# name -> (restype, argtypes)
+_LEGACY_GRID_ARGS = [C.c_double, C.c_double, C.c_int, vp, vp, vp, vp, vp,
+                     C.c_int, C.c_int, C.c_int, vp]
+SIGNATURES: dict[str, tuple] = {
+    "gridKernelCPUFullGCF": (ull, _LEGACY_GRID_ARGS),
+    "gridKernelCPUHalfGCF": (ull, _LEGACY_GRID_ARGS),
+    "deGridKernelCPUFullGCF": (ull, _LEGACY_GRID_ARGS),
+    "deGridKernelCPUHalfGCF": (ull, _LEGACY_GRID_ARGS),
+    "rcg_last_error": (C.c_int, []),
+    "rcg_last_error_string": (C.c_char_p, []),
+    "rcg_pregrid": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
+    "rcg_grid_plain_f64": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
+    "rcg_degrid_plain_f64": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load librcgrid.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RcgError(
+                    f"{LIB_PATH} is not built; run `python -m rc_amd.build` "
+                    "(the HIP library has no CPU fallback)")
+            h = C.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(h, name, None)
+                if fn is None:
+                    continue
+                fn.restype = res
+                fn.argtypes = args
+            _lib = h
+    return _lib
+
+
+def last_error() -> tuple[int, str]:
+    h = lib()
+    msg = h.rcg_last_error_string()
+    return int(h.rcg_last_error()), (msg.decode() if msg else "")
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        code, msg = last_error()
+        raise RcgError(f"{what} failed with status {status}: {msg}")

@@ -1,0 +1,126 @@
+// rcg_pregrid.hip -- FP64 uvw -> (grid cell, GCF window) pregrid on gfx950.
+//
+// Restates pregridPoint<8, mirror> (MS4/kernel/common/common.h:115-170) and
+// the per-point GCF lookup, recentring and in-bound test of the reference
+// gridder (MS4/kernel/cpu/scatter_grid/scatter_gridder_w_dependent_dyn_1p.cpp
+// :127-144).  The integer outputs must equal the reference's bit for bit, so
+// every step is a single IEEE FP64 operation with contraction switched off:
+// us = u*scale (one rounding), floor(us), us - floor(us) (exact), 8*(...)
+// (exact), floor; w/wstep (correctly rounded division) and round-half-away.
+#include <climits>
+
+#include "rcg_internal.h"
+#include "rcg_pregrid.h"
+
+namespace rcg {
+
+__global__ void __launch_bounds__(256)
+k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
+          unsigned long long *__restrict__ taps_dev, int *__restrict__ err_dev)
+{
+  const long long n = (long long)g.n_chunks * g.ts_ch;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long my_taps = 0;
+  int my_err = 0;
+  if (i < n) {
+    const int S = g.chunk_supps[i / g.ts_ch];
+    PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
+                                   t.half != 0);
+    rcg_pg rec;
+    rec.cell = -1;
+    rec.tap0 = 0;
+    rec.stride = 0;
+    rec.supp = (int16_t)S;
+    rec.flags = 0;
+    if (p.inbound) {
+      int tap0 = 0, stride = 0;
+      bool ok = resolve_window(t, p, S, &tap0, &stride);
+      if (ok) {
+        rec.cell = p.u * g.grid_pitch + p.v;
+        rec.tap0 = tap0;
+        rec.stride = (int16_t)stride;
+        rec.flags = p.conj ? PG_CONJ : 0;
+        my_taps = (unsigned long long)S * (unsigned long long)S;
+      } else {
+        rec.flags = PG_RANGE_ERR;
+        my_err = ERR_RANGE;
+      }
+    }
+    pg[i] = rec;
+  }
+  // wavefront reduction of the tap count, one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) my_taps += __shfl_xor(my_taps, off);
+  if ((threadIdx.x & 63) == 0 && my_taps && taps_dev)
+    atomicAdd(taps_dev, my_taps);
+  if (my_err && err_dev) atomicOr(err_dev, my_err);
+}
+
+// Ranges over the in-bound points: out[0..1] = min/max w-plane, out[2..3] =
+// min/max "layer plane" (the plane whose 64 layers hold the point's layer:
+// floor(index/64) for a full table, |index|/64 for a half one).  The two
+// differ when an over-sampling offset rounds up to 8 (a scaled coordinate
+// one ulp below an integer) and the layer index runs into the next plane.
+__global__ void __launch_bounds__(256)
+k_plane_range(rcg_geom g, int half, int *__restrict__ out)
+{
+  const long long n = (long long)g.n_chunks * g.ts_ch;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int lo = INT_MAX, hi = INT_MIN, llo = INT_MAX, lhi = INT_MIN;
+  if (i < n) {
+    const int S = g.chunk_supps[i / g.ts_ch];
+    PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
+                                   half != 0);
+    if (p.inbound) {
+      lo = hi = p.w_plane;
+      const int idx = half ? (p.index < 0 ? -p.index : p.index) : p.index;
+      llo = lhi = idx >= 0 ? idx / 64 : -((-idx + 63) / 64);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+    llo = min(llo, __shfl_xor(llo, off));
+    lhi = max(lhi, __shfl_xor(lhi, off));
+  }
+  if ((threadIdx.x & 63) == 0 && lo <= hi) {
+    atomicMin(out, lo);
+    atomicMax(out + 1, hi);
+    atomicMin(out + 2, llo);
+    atomicMax(out + 3, lhi);
+  }
+}
+
+rcg_status launch_plane_range(const rcg_geom *g, int half, int *out_dev,
+                              hipStream_t stream)
+{
+  const long long n = (long long)g->n_chunks * g->ts_ch;
+  if (n == 0) return RCG_OK;
+  hipLaunchKernelGGL(k_plane_range, dim3(ceil_div(n, 256)), dim3(256), 0,
+                     stream, *g, half, out_dev);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+}  // namespace rcg
+
+extern "C" rcg_status rcg_pregrid(const rcg_geom *geom, const rcg_gcf *gcf,
+                                  rcg_pg *pg, unsigned long long *taps_dev,
+                                  int *err_dev, void *stream)
+{
+  using namespace rcg;
+  clear_error();
+  if (!geom || !gcf || !pg) {
+    set_error(RCG_ERR_ARG, "rcg_pregrid: null argument");
+    return RCG_ERR_ARG;
+  }
+  rcg_status s = check_geom(geom);
+  if (s != RCG_OK) return s;
+  s = check_gcf(gcf);
+  if (s != RCG_OK) return s;
+  const long long n = (long long)geom->n_chunks * geom->ts_ch;
+  if (n == 0) return RCG_OK;
+  hipLaunchKernelGGL(k_pregrid, dim3(ceil_div(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, *geom, *gcf, pg, taps_dev, err_dev);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}

@@ -1,0 +1,136 @@
+"""Device-resident data (torch tensors on the HIP device) and the rcg_* calls.
+
+PyTorch provides the device memory and the stream; every computation is a
+librcgrid.so kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .tables import GCFTable, VisChunks
+
+PG_DTYPE = np.dtype([("cell", "<i4"), ("tap0", "<i4"), ("stride", "<i2"),
+                     ("supp", "<i2"), ("flags", "<i4")])
+PG_CONJ = 1
+PG_RANGE_ERR = 2
+
+
+def _stream(stream: torch.cuda.Stream | None) -> C.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _complex_tensor(a: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+@dataclass
+class DeviceGCF:
+    """GCF table on the device (struct rcg_gcf)."""
+    data: torch.Tensor
+    layer_off: torch.Tensor
+    layer_len: torch.Tensor
+    supps: torch.Tensor
+    center: int
+    max_w_plane: int
+    half: bool
+
+    @staticmethod
+    def from_table(t: GCFTable, device="cuda", dtype=None) -> "DeviceGCF":
+        data = t.data if dtype is None else t.data.astype(dtype)
+        return DeviceGCF(
+            data=_complex_tensor(data, device),
+            layer_off=torch.from_numpy(t.layer_off.astype(np.int64)).to(device),
+            layer_len=torch.from_numpy(t.layer_len.astype(np.int32)).to(device),
+            supps=torch.from_numpy(t.supps.astype(np.int32)).to(device),
+            center=t.center, max_w_plane=t.max_w_plane, half=t.half)
+
+    def struct(self) -> _lib.Gcf:
+        return _lib.Gcf(self.data.data_ptr(), self.layer_off.data_ptr(),
+                        self.layer_len.data_ptr(), int(self.layer_off.numel()),
+                        self.center, self.supps.data_ptr(), self.max_w_plane,
+                        int(self.half))
+
+
+@dataclass
+class DeviceGeom:
+    """Visibility geometry on the device (struct rcg_geom)."""
+    uvw: torch.Tensor          # float64 [n, 3]
+    chunk_supps: torch.Tensor  # int32 [n_chunks]
+    n_chunks: int
+    ts_ch: int
+    scale: float
+    wstep: float
+    grid_size: int
+    grid_pitch: int
+
+    @property
+    def n(self) -> int:
+        return self.n_chunks * self.ts_ch
+
+    @staticmethod
+    def from_chunks(vc: VisChunks, grid_size: int, grid_pitch: int | None = None,
+                    device="cuda") -> "DeviceGeom":
+        return DeviceGeom(
+            uvw=torch.from_numpy(np.ascontiguousarray(vc.uvw.reshape(-1, 3), np.float64)).to(device),
+            chunk_supps=torch.from_numpy(np.ascontiguousarray(vc.supps, np.int32)).to(device),
+            n_chunks=vc.n_chunks, ts_ch=vc.ts_ch, scale=float(vc.scale),
+            wstep=float(vc.wstep), grid_size=grid_size,
+            grid_pitch=grid_pitch or grid_size)
+
+    def struct(self) -> _lib.Geom:
+        return _lib.Geom(self.uvw.data_ptr(), self.chunk_supps.data_ptr(),
+                         self.n_chunks, self.ts_ch, self.scale, self.wstep,
+                         self.grid_size, self.grid_pitch)
+
+
+def pregrid(geom: DeviceGeom, gcf: DeviceGCF, stream=None):
+    """rcg_pregrid: returns (records uint8 tensor [n*16], taps tensor u64[1]
+    as int64, error word int32[1])."""
+    dev = geom.uvw.device
+    pg = torch.empty(geom.n * 16, dtype=torch.uint8, device=dev)
+    taps = torch.zeros(1, dtype=torch.int64, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    g, t = geom.struct(), gcf.struct()
+    _lib.check(_lib.lib().rcg_pregrid(C.byref(g), C.byref(t), pg.data_ptr(),
+                                      taps.data_ptr(), err.data_ptr(),
+                                      _stream(stream)), "rcg_pregrid")
+    return pg, taps, err
+
+
+def records_to_numpy(pg: torch.Tensor) -> np.ndarray:
+    return pg.cpu().numpy().view(PG_DTYPE)
+
+
+def grid_plain_f64(geom: DeviceGeom, gcf: DeviceGCF, pg: torch.Tensor,
+                   vis: torch.Tensor, grid: torch.Tensor | None = None,
+                   stream=None) -> torch.Tensor:
+    if vis.dtype != torch.complex128 or gcf.data.dtype != torch.complex128:
+        raise TypeError("FP64 path takes complex128 visibilities and taps")
+    if grid is None:
+        grid = torch.empty((geom.grid_size, geom.grid_pitch), dtype=torch.complex128,
+                           device=vis.device)
+    g, t = geom.struct(), gcf.struct()
+    _lib.check(_lib.lib().rcg_grid_plain_f64(C.byref(g), C.byref(t), pg.data_ptr(),
+                                             vis.data_ptr(), grid.data_ptr(),
+                                             _stream(stream)), "rcg_grid_plain_f64")
+    return grid
+
+
+def degrid_plain_f64(geom: DeviceGeom, gcf: DeviceGCF, pg: torch.Tensor,
+                     grid: torch.Tensor, vis: torch.Tensor | None = None,
+                     stream=None) -> torch.Tensor:
+    if grid.dtype != torch.complex128 or gcf.data.dtype != torch.complex128:
+        raise TypeError("FP64 path takes a complex128 grid and taps")
+    if vis is None:
+        vis = torch.empty(geom.n, dtype=torch.complex128, device=grid.device)
+    g, t = geom.struct(), gcf.struct()
+    _lib.check(_lib.lib().rcg_degrid_plain_f64(C.byref(g), C.byref(t), pg.data_ptr(),
+                                               grid.data_ptr(), vis.data_ptr(),
+                                               _stream(stream)), "rcg_degrid_plain_f64")
+    return vis
