@@ -1,0 +1,229 @@
+"""Benchmark: w-projection gridding + degridding on one MI355X.
+
+Workload: 4096 x 4096 complex64 grid, 32 |w|-planes (supports 8..64, 8x
+oversampling, half GCF table), 9,987,840 points from 256 antennas tracked
+over 4 h at declination -45 deg, 1.4 GHz, complex64 visibilities N(0,1).
+All inputs are resident in HBM before timing starts.
+
+One step = pregrid (FP64) + uv-tile binning + FP32 gridding + FP32
+degridding of every point (rc_amd tiled path).  value = points / step time
+(Mvis/s gridded+degridded).  The FP64 plain path (pregrid + FP64-atomic
+gridder + FP64 degridder, one launch each) is timed the same way as the
+comparison leg, and the FP32 results are compared with it (the FP64 path is
+within 1e-12 of the reference's CPU gridder).
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD published
+FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tile", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=512)
+    ap.add_argument("--n-times", type=int, default=306)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(vc, gcf, N, budget_s: float) -> dict:
+    """The reference's own CPU gridder + degridder (compiled from its sources
+    with its release flags, oracle/_ref/libms4ref_fast.so), else the oracle
+    restatement, on a prefix sample of the same workload; OpenMP threads =
+    the host cores available to this process (at most 16)."""
+    from oracle import oracle as O
+    from rc_amd.tables import VisChunks
+    threads = min(16, len(os.sched_getaffinity(0)))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    kind = "reference" if O.ref_available(fast=True) else "port"
+    impl = "ref_fast" if kind == "reference" else "oracle"
+    gcf128 = gcf.astype(np.complex128)
+
+    def sample(m):
+        return VisChunks(uvw=vc.uvw[:m], vis=vc.vis[:m].astype(np.complex128),
+                         supps=vc.supps[:m], scale=vc.scale, wstep=vc.wstep)
+
+    def run(m):
+        s = sample(m)
+        t0 = time.perf_counter()
+        g, taps = O.grid(s, gcf128, N, impl=impl, nthreads=threads)
+        t1 = time.perf_counter()
+        O.degrid(s, gcf128, g, impl=impl, nthreads=threads)
+        t2 = time.perf_counter()
+        return t2 - t0, t1 - t0, t2 - t1, taps
+
+    m = 100_000
+    dt, _, _, _ = run(m)
+    m2 = int(min(vc.n, max(m, m * budget_s / max(dt, 1e-3))))
+    dt, tg, td, taps = run(m2)
+    return {"value": round(m2 / dt / 1e6, 4), "unit": "Mvis/s", "cores": threads,
+            "kind": kind,
+            "sample": f"first {m2} points of the workload (taps {taps}), full 4096^2 "
+                      f"complex128 grid, grid {tg:.2f} s + degrid {td:.2f} s, "
+                      f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the path "
+                          "shards across GPUs by design (see DESIGN.md), but no "
+                          "multi-GPU measurement is made", "n_gpus": args.gpus}))
+        return 2
+    import torch
+    from rc_amd import device as D
+    from rc_amd import synth
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device visible"}))
+        return 1
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    vc, gcf, N = synth.wproj(n_times=args.n_times)
+    log(f"workload: {vc.n} points, {time.time() - t0:.1f} s to generate")
+    geom = D.DeviceGeom.from_chunks(vc, N)
+    dg32 = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
+    vis32 = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).to(dev)
+    plan = D.TiledPlan(geom, tile=args.tile, chunk=args.chunk, max_support=64)
+    grid32 = torch.empty((N, N), dtype=torch.complex64, device=dev)
+    out32 = torch.empty(vc.n, dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step32(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        plan.bin(dg32)
+        if ev is not None:
+            ev[1].record(stream)
+        plan.grid(dg32, vis32, grid32)
+        if ev is not None:
+            ev[2].record(stream)
+        plan.degrid(dg32, grid32, out32)
+        if ev is not None:
+            ev[3].record(stream)
+
+    for _ in range(args.warmup):
+        step32()
+    torch.cuda.synchronize()
+    st = plan.stats()
+    if st["err"]:
+        print(json.dumps({"error": f"binning error word {st['err']}"}))
+        return 1
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step32(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    t_bin = np.mean([e[0].elapsed_time(e[1]) for e in evs])
+    t_grid = np.mean([e[1].elapsed_time(e[2]) for e in evs])
+    t_degrid = np.mean([e[2].elapsed_time(e[3]) for e in evs])
+    value = vc.n / (ms_step / 1e3) / 1e6
+    log(f"fp32 tiled: {ms_step:.3f} ms/step (bin {t_bin:.3f} grid {t_grid:.3f} "
+        f"degrid {t_degrid:.3f} ms) -> {value:.1f} Mvis/s; stats {st}")
+
+    extra = {}
+    taps = st["taps"]
+    if not args.no_fp64:
+        dg64 = D.DeviceGCF.from_table(gcf, dtype=np.complex128)
+        vis64 = vis32.to(torch.complex128)
+        grid64 = torch.empty((N, N), dtype=torch.complex128, device=dev)
+        out64 = torch.empty(vc.n, dtype=torch.complex128, device=dev)
+
+        def step64():
+            pg, _, _ = D.pregrid(geom, dg64)
+            D.grid_plain_f64(geom, dg64, pg, vis64, grid64)
+            D.degrid_plain_f64(geom, dg64, pg, grid64, out64)
+
+        step64()
+        torch.cuda.synchronize()
+        n64 = max(2, args.steps // 3)
+        t0 = time.perf_counter()
+        for _ in range(n64):
+            step64()
+        torch.cuda.synchronize()
+        ms64 = (time.perf_counter() - t0) / n64 * 1e3
+        v64 = vc.n / (ms64 / 1e3) / 1e6
+        # errors of the FP32 path against FP64 on the same complex64 inputs
+        plan.grid(dg32, vis32, grid32)
+        g_err = float(torch.linalg.vector_norm(grid32.to(torch.complex128) - grid64)
+                      / torch.linalg.vector_norm(grid64))
+        g32in = grid64.to(torch.complex64)
+        plan.degrid(dg32, g32in, out32)
+        D.degrid_plain_f64(geom, dg64, D.pregrid(geom, dg64)[0], g32in.to(torch.complex128), out64)
+        d_err = float(torch.linalg.vector_norm(out32.to(torch.complex128) - out64)
+                      / torch.linalg.vector_norm(out64))
+        torch.cuda.synchronize()
+        log(f"fp64 plain: {ms64:.3f} ms/step -> {v64:.1f} Mvis/s; fp32 grid rel L2 "
+            f"{g_err:.2e}, degrid rel L2 {d_err:.2e}")
+        extra = {"fp64_plain_mvis_s": round(v64, 3), "fp64_plain_ms_per_step": round(ms64, 3),
+                 "speedup_vs_fp64_plain": round(value / v64, 3),
+                 "error_vs_fp64": {"grid_rel_l2": g_err, "degrid_rel_l2": d_err}}
+
+    # roofline of the dominant kernel (algorithmic bytes per launch: points x
+    # (24 B uvw + 8 B vis) + 8 B per tap + 8 B per grid cell)
+    b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
+    t_dom = max(t_grid, t_degrid)
+    dom = "grid" if t_grid >= t_degrid else "degrid"
+    achieved = b_dir / (t_dom / 1e3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "valu_tflops": round(8 * taps / (t_dom / 1e3) / 1e12, 3),
+            "valu_frac": round(8 * taps / (t_dom / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(vc, gcf, N, args.cpu_seconds)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:  # reported, not fatal
+            cpu = {"error": repr(e)}
+
+    line = {
+        "metric": "Mvis/s gridded+degridded, 4096² 32 w-planes: FP32 batched vs FP64 plain @1e-5",
+        "value": round(value, 3), "unit": "Mvis/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64/f32 (f64 pregrid)",
+        "data": "synthetic",
+        "config": {"workload": "w-projection grid+degrid, 4096x4096, 32 w-planes, "
+                               "supports 8-64, 8x oversampling",
+                   "points": vc.n, "taps_per_direction": taps, "tile": args.tile,
+                   "chunk": args.chunk},
+        "phases_ms": {"pregrid_bin": round(float(t_bin), 4), "grid": round(float(t_grid), 4),
+                      "degrid": round(float(t_degrid), 4)},
+        "roofline": roof, "cpu_baseline": cpu, **extra,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -141,7 +141,8 @@ const char *rcg_last_error_string(void);
  * signed (w_plane*8+over_u)*8+over_v) is L + center for a full table and
  * |L| for a half table (center must be 0).  supps[p + max_w_plane] is the
  * support of w-plane p for -max_w_plane <= p <= max_w_plane.
- * data is rcg_c128 for the FP64 entry points and rcg_c64 for FP32 ones. */
+ * data is rcg_c128 for the FP64 entry points and rcg_c64 for FP32 ones;
+ * the FP32 entry points require n_taps * 8 < 2^31. */
 typedef struct rcg_gcf {
   const void *data;
   const int64_t *layer_off;
@@ -151,6 +152,7 @@ typedef struct rcg_gcf {
   const int32_t *supps;
   int32_t max_w_plane;
   int32_t half;
+  int64_t n_taps;     /* complex elements in data (bounds of tap reads) */
 } rcg_gcf;
 
 /* Visibility geometry in device memory: n_chunks chunks of ts_ch points,
@@ -193,6 +195,32 @@ rcg_status rcg_grid_plain_f64(const rcg_geom *geom, const rcg_gcf *gcf,
 rcg_status rcg_degrid_plain_f64(const rcg_geom *geom, const rcg_gcf *gcf,
                                 const rcg_pg *pg, const rcg_c128 *grid,
                                 rcg_c128 *vis, void *stream);
+
+/* Tiled FP32 path.  A plan owns the device scratch for one geometry (point
+ * count and grid shape): pregrid records, per-tile bins of (point, tile)
+ * pairs and work-item tables.  tile = 32 or 64 grid cells; chunk = maximum
+ * pairs per work item; max_support bounds every chunk support.
+ * rcg_tiled_bin pregrids and bins the points (device only, no host sync);
+ * grid/degrid then run on complex64 visibilities, taps (gcf->data) and grid,
+ * accumulating in FP32.  Every call is asynchronous on `stream`; calls that
+ * share a plan must be ordered on one stream.  No call allocates. */
+typedef struct rcg_tiled_plan rcg_tiled_plan;
+rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile, int chunk,
+                                 int max_support, rcg_tiled_plan **plan);
+rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *plan);
+rcg_status rcg_tiled_bin(rcg_tiled_plan *plan, const rcg_geom *geom,
+                         const rcg_gcf *gcf, void *stream);
+rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                              const rcg_c64 *vis, rcg_c64 *grid, void *stream);
+rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                                const rcg_c64 *grid, rcg_c64 *vis,
+                                void *stream);
+/* Synchronises `stream`; returns the in-bound tap count, pair and work-item
+ * counts and the device error word of the last rcg_tiled_bin (bit 0: a point
+ * referenced a layer outside the table; bit 1: pair buffer overflow). */
+rcg_status rcg_tiled_stats(rcg_tiled_plan *plan, unsigned long long *taps,
+                           long long *n_pairs, long long *n_items, int *err,
+                           void *stream);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ class Gcf(C.Structure):
         ("supps", vp),
         ("max_w_plane", C.c_int32),
         ("half", C.c_int32),
+        ("n_taps", C.c_int64),
     ]
 
 
@@ -65,6 +66,14 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_pregrid": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
     "rcg_grid_plain_f64": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
     "rcg_degrid_plain_f64": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
+    "rcg_tiled_plan_create": (C.c_int, [C.POINTER(Geom), C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(vp)]),
+    "rcg_tiled_plan_destroy": (C.c_int, [vp]),
+    "rcg_tiled_bin": (C.c_int, [vp, C.POINTER(Geom), C.POINTER(Gcf), vp]),
+    "rcg_grid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
+    "rcg_degrid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
+    "rcg_tiled_stats": (C.c_int, [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_longlong),
+                                  C.POINTER(C.c_longlong), C.POINTER(C.c_int), vp]),
 }
 
 

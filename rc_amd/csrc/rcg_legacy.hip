@@ -205,6 +205,7 @@ ull legacy_call(const LegacyArgs &a)
   t.supps = d_tsupp.as<int32_t>();
   t.max_w_plane = M;
   t.half = a.half ? 1 : 0;
+  t.n_taps = (int64_t)hdata.size();
 
   if (rcg_pregrid(&g, &t, d_pg.as<rcg_pg>(), d_taps.as<unsigned long long>(),
                   d_err.as<int>(), st) != RCG_OK)

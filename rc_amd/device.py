@@ -54,7 +54,7 @@ class DeviceGCF:
         return _lib.Gcf(self.data.data_ptr(), self.layer_off.data_ptr(),
                         self.layer_len.data_ptr(), int(self.layer_off.numel()),
                         self.center, self.supps.data_ptr(), self.max_w_plane,
-                        int(self.half))
+                        int(self.half), int(self.data.numel()))
 
 
 @dataclass
@@ -134,3 +134,79 @@ def degrid_plain_f64(geom: DeviceGeom, gcf: DeviceGCF, pg: torch.Tensor,
                                                grid.data_ptr(), vis.data_ptr(),
                                                _stream(stream)), "rcg_degrid_plain_f64")
     return vis
+
+
+class TiledPlan:
+    """A tiled FP32 gridding plan (rcg_tiled_plan): pregrid + uv-tile bins for
+    one geometry, reused by grid and degrid.  Device scratch is owned by the
+    library and released by close()."""
+
+    def __init__(self, geom: DeviceGeom, tile: int = 64, chunk: int = 512,
+                 max_support: int | None = None):
+        self.geom = geom
+        if max_support is None:
+            max_support = int(geom.chunk_supps.max().item()) if geom.n_chunks else 1
+        self.tile, self.chunk, self.max_support = tile, chunk, max(1, max_support)
+        h = C.c_void_p()
+        g = geom.struct()
+        _lib.check(_lib.lib().rcg_tiled_plan_create(C.byref(g), tile, chunk,
+                                                    self.max_support, C.byref(h)),
+                   "rcg_tiled_plan_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().rcg_tiled_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bin(self, gcf: DeviceGCF, stream=None) -> None:
+        g, t = self.geom.struct(), gcf.struct()
+        _lib.check(_lib.lib().rcg_tiled_bin(self._h, C.byref(g), C.byref(t),
+                                            _stream(stream)), "rcg_tiled_bin")
+
+    def grid(self, gcf: DeviceGCF, vis: torch.Tensor, grid: torch.Tensor | None = None,
+             stream=None) -> torch.Tensor:
+        if vis.dtype != torch.complex64 or gcf.data.dtype != torch.complex64:
+            raise TypeError("FP32 path takes complex64 visibilities and taps")
+        if vis.numel() != self.geom.n or not vis.is_contiguous():
+            raise ValueError("vis must be a contiguous tensor of n points")
+        if grid is None:
+            grid = torch.empty((self.geom.grid_size, self.geom.grid_pitch),
+                               dtype=torch.complex64, device=vis.device)
+        t = gcf.struct()
+        _lib.check(_lib.lib().rcg_grid_tiled_f32(self._h, C.byref(t), vis.data_ptr(),
+                                                 grid.data_ptr(), _stream(stream)),
+                   "rcg_grid_tiled_f32")
+        return grid
+
+    def degrid(self, gcf: DeviceGCF, grid: torch.Tensor, vis: torch.Tensor | None = None,
+               stream=None) -> torch.Tensor:
+        if grid.dtype != torch.complex64 or gcf.data.dtype != torch.complex64:
+            raise TypeError("FP32 path takes a complex64 grid and taps")
+        if tuple(grid.shape) != (self.geom.grid_size, self.geom.grid_pitch) or \
+                not grid.is_contiguous():
+            raise ValueError("grid shape does not match the plan")
+        if vis is None:
+            vis = torch.empty(self.geom.n, dtype=torch.complex64, device=grid.device)
+        t = gcf.struct()
+        _lib.check(_lib.lib().rcg_degrid_tiled_f32(self._h, C.byref(t), grid.data_ptr(),
+                                                   vis.data_ptr(), _stream(stream)),
+                   "rcg_degrid_tiled_f32")
+        return vis
+
+    def stats(self, stream=None) -> dict:
+        taps = C.c_ulonglong()
+        pairs = C.c_longlong()
+        items = C.c_longlong()
+        err = C.c_int()
+        _lib.check(_lib.lib().rcg_tiled_stats(self._h, C.byref(taps), C.byref(pairs),
+                                              C.byref(items), C.byref(err),
+                                              _stream(stream)), "rcg_tiled_stats")
+        return {"taps": taps.value, "pairs": pairs.value, "items": items.value,
+                "err": err.value}
