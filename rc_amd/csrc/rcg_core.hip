@@ -44,6 +44,11 @@ rcg_status check_geom(const rcg_geom *g)
               g->grid_size, g->grid_pitch);
     return RCG_ERR_ARG;
   }
+  if ((long long)g->n_chunks * g->ts_ch > 0x7fffffffLL) {
+    set_error(RCG_ERR_ARG, "%lld points exceed 2^31 - 1",
+              (long long)g->n_chunks * g->ts_ch);
+    return RCG_ERR_ARG;
+  }
   if ((long long)g->n_chunks * g->ts_ch > 0 && (!g->uvw || !g->chunk_supps)) {
     set_error(RCG_ERR_ARG, "null uvw or chunk support array");
     return RCG_ERR_ARG;

@@ -83,6 +83,10 @@ ull legacy_call(const LegacyArgs &a)
   zero_outputs();
   const long long n = (long long)a.baselines * a.ts_ch;
   if (n == 0) return 0;
+  if (n > 0x7fffffffLL) {
+    set_error(RCG_ERR_ARG, "%lld points exceed 2^31 - 1", n);
+    return 0;
+  }
   if (!a.bl_supps || !a.uvw || !a.gcf || !a.gcf_supps) {
     set_error(RCG_ERR_ARG, "null support, uvw or GCF table");
     return 0;

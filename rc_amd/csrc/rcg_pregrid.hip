@@ -18,12 +18,13 @@ __global__ void __launch_bounds__(256)
 k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
           unsigned long long *__restrict__ taps_dev, int *__restrict__ err_dev)
 {
-  const long long n = (long long)g.n_chunks * g.ts_ch;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // point counts are below 2^31 (check_geom), so 32-bit index arithmetic
+  const int n = g.n_chunks * g.ts_ch;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long my_taps = 0;
   int my_err = 0;
   if (i < n) {
-    const int S = g.chunk_supps[i / g.ts_ch];
+    const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
     PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
                                    t.half != 0);
     rcg_pg rec;
@@ -63,11 +64,11 @@ k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
 __global__ void __launch_bounds__(256)
 k_plane_range(rcg_geom g, int half, int *__restrict__ out)
 {
-  const long long n = (long long)g.n_chunks * g.ts_ch;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = g.n_chunks * g.ts_ch;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int lo = INT_MAX, hi = INT_MIN, llo = INT_MAX, lhi = INT_MIN;
   if (i < n) {
-    const int S = g.chunk_supps[i / g.ts_ch];
+    const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
     PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
                                    half != 0);
     if (p.inbound) {

@@ -30,18 +30,35 @@
 
 namespace rcg {
 
-// Pair record: one (point, tile) incidence, 16 bytes.
-//   x: u0 << 16 | v0 (window top-left cell, both < 32768)
-//   y: tap0 (element offset of the recentred window)
-//   z: stride | supp << 16 | conj << 31
+// Pair record: one (point, tile) incidence with the geometry of the
+// window/tile intersection precomputed, 16 bytes.
+//   x: r0 | c0 << 6 | (w-1) << 12 | (rows-1) << 18 | conj << 30 | whole << 31
+//      r0, c0: first tile row/column of the intersection (< 64)
+//      w:      intersection width (1..64)
+//      rows:   64 / w, the rows one 64-lane slab covers
+//      whole:  the window lies entirely in this tile
+//   y: tap offset (elements) of the intersection's first cell
+//   z: stride | nfull << 16 | hrem << 23
+//      stride: layer row stride (elements, < 65536)
+//      nfull:  slabs whose rows are all inside (h / rows)
+//      hrem:   rows of the last, partial slab (h % rows)
 //   w: point index
-__device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int point)
+__device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int point,
+                                          int tile_r, int tile_c, int T)
 {
+  const int S = r.supp, stride = r.stride;
+  const int tu0 = tile_r * T, tv0 = tile_c * T;
+  const int ra = max(u0, tu0), rb = min(u0 + S, tu0 + T);
+  const int ca = max(v0, tv0), cb = min(v0 + S, tv0 + T);
+  const int h = rb - ra, w = cb - ca;
+  const int rows = 64 / w;
+  const int nfull = h / rows, hrem = h - nfull * rows;
+  const int whole = (h == S) && (w == S);
   int4 p;
-  p.x = (u0 << 16) | v0;
-  p.y = r.tap0;
-  p.z = (int)(uint16_t)r.stride | ((int)r.supp << 16) |
-        ((r.flags & PG_CONJ) ? (int)0x80000000 : 0);
+  p.x = (ra - tu0) | ((ca - tv0) << 6) | ((w - 1) << 12) | ((rows - 1) << 18) |
+        ((r.flags & PG_CONJ) ? (1 << 30) : 0) | (whole ? (int)0x80000000 : 0);
+  p.y = r.tap0 + (ra - u0) * stride + (ca - v0);
+  p.z = stride | (nfull << 16) | (hrem << 23);
   p.w = point;
   return p;
 }
@@ -209,14 +226,13 @@ k_bin_fill(const rcg_pg *__restrict__ pg, long long n, int pitch, int lt,
     int u0, v0, tu0, tu1, tv0, tv1;
     point_tiles(r, pitch, lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
     if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > max_pairs_pt) continue;
-    const int4 pr = make_pair(u0, v0, r, (int)i);
     int j = 0;
     for (int a = tu0; a <= tu1; a++)
       for (int b = tv0; b <= tv1; b++, j++) {
         const long long pos =
             (long long)hist[a * ntx + b] +
             rank[((k * BIN_THREADS) + threadIdx.x) * max_pairs_pt + j];
-        if (pos < pair_cap) pairs[pos] = pr;
+        if (pos < pair_cap) pairs[pos] = make_pair(u0, v0, r, (int)i, a, b, 1 << lt);
         else atomicOr(err, ERR_CAPACITY);
       }
   }
@@ -283,7 +299,7 @@ __device__ __forceinline__ bool item_range(const TiledDev &d, ItemRange *ir)
 // LDS access at a pad cell.  JOB_U steps are issued before they are
 // consumed, so JOB_U tap loads per lane are in flight.
 
-constexpr int JOB_U = 4;
+
 constexpr unsigned TAP_OOB = 0x7ffffff0u;  // byte offset past any tap table
 
 __device__ __forceinline__ int rl(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
@@ -300,69 +316,115 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t tap_rsrc(const float2 *taps, l
                                            (int)bytes, 0x00020000);
 }
 
-__device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned off)
+// voff: per-lane byte offset (checked against the table size); soff:
+// uniform byte offset added to it.
+__device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
 {
-  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
   return make_float2(__int_as_float(v[0]), __int_as_float(v[1]));
 }
 
-struct Walk {
-  // uniform
-  int h, w, K, point, whole, conj;
-  int dr, dc, tstep, twrap, lstep, lwrap;
-  // per lane: row, column, tap byte offset, LDS element index
-  int r, c, tb, li;
+// Per pair: the lanes form `rows` rows of `w` columns (lane l at row
+// l / w, column l % w; lanes past rows*w idle); slab s covers intersection
+// rows [s*rows, (s+1)*rows), the last one possibly partial.  A lane's tap
+// offset for slab s is its slab-0 offset plus the uniform s*rows*stride (the
+// buffer load's scalar offset); its LDS index advances by rows*LS per slab.
+// A lane outside the intersection in a slab loads its tap at an
+// out-of-range offset (which reads zero) and points its LDS access at the
+// copy's dummy cell.
+//
+// Latency: a wavefront loads the records (and visibilities) of 64 pairs at
+// once, one per lane, double-buffered one batch ahead, and broadcasts them
+// with readlane; the taps of a pair's first group of up to four slabs are
+// loaded while the previous pair's LDS updates run.
+constexpr int GRP = 4;
+
+struct Pair {
+  int h, rows, nslabs, point, whole, conj;
+  int lr;          // lane row (idle lanes: 64)
+  unsigned tb0;    // tap byte offset of the lane's slab-0 cell
+  int li0;         // LDS index of the lane's slab-0 cell
+  int tstep;       // tap byte step per slab (uniform)
+  int lstep;       // LDS index step per slab
 };
 
-template <int LS>
-__device__ __forceinline__ Walk walk_pair(const int4 &my, int j, int tu0, int tv0,
-                                          int T, int lane)
+template <int LS, int DUMMY>
+__device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int lane)
 {
-  Walk k;
-  const int px = rl(my.x, j), py = rl(my.y, j), pz = rl(my.z, j);
-  k.point = rl(my.w, j);
-  const int u0 = px >> 16, v0 = px & 0xffff;
-  const int S = (pz >> 16) & 0x7fff;
+  Pair p;
   const int stride = pz & 0xffff;
-  k.conj = pz < 0;
-  const int ra = max(u0, tu0), rb = min(u0 + S, tu0 + T);
-  const int ca = max(v0, tv0), cb = min(v0 + S, tv0 + T);
-  k.h = rb - ra;
-  k.w = cb - ca;
-  k.whole = (k.h == S) && (k.w == S);
-  k.K = (k.h * k.w + 63) >> 6;
-  // integer quotients by w through the approximate reciprocal: (x + 0.5)/w
-  // is at least 0.5/w from an integer, far above the rounding error
-  const float rw = __builtin_amdgcn_rcpf((float)k.w);
-  k.dr = (int)(64.5f * rw);
-  k.dc = 64 - k.dr * k.w;
-  k.tstep = (k.dr * stride + k.dc) * 8;
-  k.twrap = (stride - k.w) * 8;
-  k.lstep = k.dr * LS + k.dc;
-  k.lwrap = LS - k.w;
-  k.r = (int)(((float)lane + 0.5f) * rw);
-  k.c = lane - k.r * k.w;
-  k.tb = (py + (ra - u0 + k.r) * stride + (ca - v0) + k.c) * 8;
-  k.li = (ra - tu0 + k.r) * LS + (ca - tv0) + k.c;
-  return k;
+  const int nfull = (pz >> 16) & 127;
+  const int hrem = (pz >> 23) & 63;
+  p.point = pw;
+  const int r0 = px & 63, c0 = (px >> 6) & 63;
+  const int w = ((px >> 12) & 63) + 1;
+  p.rows = ((px >> 18) & 63) + 1;
+  p.h = nfull * p.rows + hrem;
+  p.nslabs = nfull + (hrem ? 1 : 0);
+  p.conj = (px >> 30) & 1;
+  p.whole = px < 0;
+  // lane / w through the approximate reciprocal: (lane + 0.5) / w is at
+  // least 0.5 / w away from an integer, far above its rounding error
+  const int lr = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+  const int lc = lane - lr * w;
+  const bool idle = lr >= p.rows;
+  p.lr = idle ? 64 : lr;
+  p.tb0 = idle ? TAP_OOB : (unsigned)(py + lr * stride + lc) * 8u;
+  p.li0 = idle ? DUMMY : (r0 + lr) * LS + c0 + lc;
+  p.tstep = p.rows * stride * 8;
+  p.lstep = p.rows * LS;
+  return p;
 }
 
-__device__ __forceinline__ void walk_step(Walk &k)
+// taps of slabs [s0, s0 + GRP) of pair p (zero outside the intersection)
+__device__ __forceinline__ void load_group(float2 (&t)[GRP], const Pair &p, int s0,
+                                           __amdgpu_buffer_rsrc_t rsrc)
 {
-  k.c += k.dc;
-  const bool m = k.c >= k.w;
-  k.c = m ? k.c - k.w : k.c;
-  k.r += k.dr + (m ? 1 : 0);
-  k.tb += k.tstep + (m ? k.twrap : 0);
-  k.li += k.lstep + (m ? k.lwrap : 0);
+#pragma unroll
+  for (int u = 0; u < GRP; u++) {
+    const int rs = (s0 + u) * p.rows;
+    const bool act = p.lr < p.h - rs;
+    t[u] = tap_load(rsrc, act ? p.tb0 : TAP_OOB, (s0 + u) * p.tstep);
+  }
+}
+
+__device__ __forceinline__ int group_lds(const Pair &p, int s, int dummy)
+{
+  const int rs = s * p.rows;
+  return p.lr < p.h - rs ? p.li0 + s * p.lstep : dummy;
+}
+
+// Stream of work for one wavefront: its pairs ir.begin + wave + 4k, each cut
+// into groups of GRP slabs.  Records (and visibilities) come 64 at a time,
+// one per lane, double-buffered a batch ahead and broadcast with readlane.
+struct Stream {
+  int npw;       // pairs of this wavefront
+  int k;         // current pair
+  int s0;        // first slab of the current group
+  int4 rec, rec_n;
+  float2 mv, mv_n;
+};
+
+template <bool VIS>
+__device__ __forceinline__ void stream_batch(const TiledDev &d, const float2 *__restrict__ vis,
+                                             const ItemRange &ir, int wave, int lane, int npw,
+                                             int b, int4 &rec, float2 &v)
+{
+  const int k = b * 64 + lane;
+  rec = make_int4(0, 0, 0, 0);
+  v = make_float2(0.f, 0.f);
+  if (k < npw) {
+    rec = d.pairs[ir.begin + wave + 4 * k];
+    if (VIS) v = vis[rec.w];
+  }
 }
 
 // ---------------------------------------------------------------------------
 // gridding: each wavefront owns a private copy of the tile in LDS, so the
-// read-modify-write of a cell needs no atomic: the cells of one pair's steps
-// are distinct, and a wavefront's LDS operations complete in program order;
-// the four copies are summed at the flush.  Column T of every row is padding
-// that absorbs the (zero) updates of lanes past the end of a pair.
+// read-modify-write of a cell needs no atomic: within one pair every lane
+// and slab has its own cell (lanes outside the intersection only ever add
+// zero to the dummy cell), and a wavefront's LDS operations complete in
+// program order; the four copies are summed at the flush.
 
 template <int T>
 __global__ void __launch_bounds__(256)
@@ -371,56 +433,99 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
              float2 *__restrict__ grid)
 {
   constexpr int LS = T + 1;
-  constexpr int CP = T * LS;  // one copy
+  constexpr int CP = T * LS + 2;  // one copy (+ dummy cell)
+  constexpr int DUMMY = T * LS;
   __shared__ float2 acc[4 * CP];
   ItemRange ir;
   if (!item_range(d, &ir)) return;
   for (int q = threadIdx.x; q < 4 * CP; q += 256) acc[q] = make_float2(0.f, 0.f);
   __syncthreads();
-  const __amdgpu_buffer_rsrc_t rs = tap_rsrc(taps, n_taps);
-  const int tu0 = (ir.tile / d.ntx) * T, tv0 = (ir.tile % d.ntx) * T;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   float2 *my_acc = acc + wave * CP;
-  const int pad = (lane & 31) * LS + T;
-  for (int base = ir.begin + wave * 64; base < ir.end; base += 256) {
-    const int nb = min(64, ir.end - base);
-    int4 my = make_int4(0, 0, 0, 0);
-    float2 mv = make_float2(0.f, 0.f);
-    if (lane < nb) {
-      my = d.pairs[base + lane];
-      mv = vis[my.w];
-    }
-    for (int j = 0; j < nb; j++) {
-      Walk k = walk_pair<LS>(my, j, tu0, tv0, T, lane);
-      // contribution of tap t (conjugated for mirrored w):
-      //   re = a*t.x + b*t.y, im = c*t.y + e*t.x
-      const float vx = rlf(mv.x, j), vy = rlf(mv.y, j);
-      const float a = vx, b = k.conj ? vy : -vy, c = k.conj ? -vx : vx, e = vy;
-      for (int s0 = 0; s0 < k.K; s0 += JOB_U) {
-        int li[JOB_U];
-        float2 t[JOB_U];
-#pragma unroll
-        for (int u = 0; u < JOB_U; u++) {
-          const bool act = (s0 + u < k.K) && (k.r < k.h);
-          li[u] = act ? k.li : pad;
-          t[u] = tap_load(rs, act ? (unsigned)k.tb : TAP_OOB);
-          walk_step(k);
+  const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
+  if (npw > 0) {
+    int4 rec, rec_n;
+    float2 mv, mv_n;
+    stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv);
+    stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n);
+    int k = 0, s0 = 0;
+    Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane);
+    float vx = rlf(mv.x, 0), vy = rlf(mv.y, 0);
+    float2 ta[GRP], tb[GRP];
+    load_group(ta, p, 0, rsrc);
+    bool more = true;
+    // One group: choose the next group (the same pair's next slabs, or the
+    // next pair's first), issue its taps into tn (past the end pn == p: a
+    // harmless reload), then update the current group's cells from tc while
+    // they fly.  Called alternately with the two buffers, so no register
+    // copy forces a wait on the loads in flight.
+    auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
+      Pair pn = p;
+      float vxn = vx, vyn = vy;
+      int kn = k, sn = s0 + GRP;
+      if (sn >= p.nslabs) {
+        kn = k + 1;
+        sn = 0;
+        if (kn < npw) {
+          const int j = kn & 63;
+          if (j == 0) {
+            rec = rec_n;
+            mv = mv_n;
+            stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n);
+          }
+          pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j),
+                                      lane);
+          vxn = rlf(mv.x, j);
+          vyn = rlf(mv.y, j);
         }
-        float2 x[JOB_U];
-#pragma unroll
-        for (int u = 0; u < JOB_U; u++) x[u] = my_acc[li[u]];
-#pragma unroll
-        for (int u = 0; u < JOB_U; u++) {
-          x[u].x = fmaf(a, t[u].x, fmaf(b, t[u].y, x[u].x));
-          x[u].y = fmaf(c, t[u].y, fmaf(e, t[u].x, x[u].y));
-        }
-#pragma unroll
-        for (int u = 0; u < JOB_U; u++) my_acc[li[u]] = x[u];
       }
+      more = kn < npw;
+      load_group(tn, pn, sn, rsrc);
+      __builtin_amdgcn_sched_barrier(0);
+      const float a = vx, b = p.conj ? vy : -vy, c = p.conj ? -vx : vx, e = vy;
+      int li[GRP];
+      float2 x[GRP];
+#pragma unroll
+      for (int u = 0; u < GRP; u++) li[u] = group_lds(p, s0 + u, DUMMY);
+      // cells of one pair are distinct, so the reads may precede the writes
+#pragma unroll
+      for (int u = 0; u < GRP; u++) x[u] = my_acc[li[u]];
+#pragma unroll
+      for (int u = 0; u < GRP; u++) {
+        x[u].x = fmaf(a, tc[u].x, fmaf(b, tc[u].y, x[u].x));
+        x[u].y = fmaf(c, tc[u].y, fmaf(e, tc[u].x, x[u].y));
+      }
+#pragma unroll
+      for (int u = 0; u < GRP; u++) my_acc[li[u]] = x[u];
+      p = pn;
+      vx = vxn;
+      vy = vyn;
+      k = kn;
+      s0 = sn;
+    };
+    while (true) {
+      body(ta, tb);
+      if (!more) break;
+      body(tb, ta);
+      if (!more) break;
+    }
+    // Never true (chunk >= 1): a use of both buffers on the loop exits keeps
+    // the compiler from sinking the prefetch loads past the exit tests.
+    if (d.chunk < 0) {
+      float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < GRP; u++) {
+        z.x += ta[u].x + tb[u].x;
+        z.y += ta[u].y + tb[u].y;
+      }
+      my_acc[lane] = z;
     }
   }
   __syncthreads();
   // flush: sum the four copies
+  const int tu0 = (ir.tile / d.ntx) * T, tv0 = (ir.tile % d.ntx) * T;
   const int gsz = d.grid_size;
   for (int q = threadIdx.x; q < T * T; q += 256) {
     const int r = q / T, cc = q - (q / T) * T;
@@ -442,12 +547,27 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
 
 // ---------------------------------------------------------------------------
 // degridding: the grid tile is staged once in LDS (read-only); per pair the
-// lanes accumulate four partial products and reduce across the wavefront.
+// lanes accumulate four partial products, then one wavefront reduction.
 
+// Sum over the 64 lanes with DPP: prefix sums within each row of 16 lanes
+// (row_shr 1, 2, 4, 8 with zero fill), then the four row totals (lanes 15,
+// 31, 47, 63) are read into scalars.
 __device__ __forceinline__ float wave_sum(float x)
 {
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  return x;
+  int v = __float_as_int(x);
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+  v = __float_as_int(x);
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+  v = __float_as_int(x);
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+  v = __float_as_int(x);
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+  v = __float_as_int(x);
+  const float a = __int_as_float(__builtin_amdgcn_readlane(v, 15));
+  const float b = __int_as_float(__builtin_amdgcn_readlane(v, 31));
+  const float c = __int_as_float(__builtin_amdgcn_readlane(v, 47));
+  const float e = __int_as_float(__builtin_amdgcn_readlane(v, 63));
+  return (a + b) + (c + e);
 }
 
 template <int T>
@@ -457,7 +577,8 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
                float2 *__restrict__ vis)
 {
   constexpr int LS = T + 1;
-  __shared__ float2 tile[T * LS];
+  constexpr int DUMMY = T * LS;
+  __shared__ float2 tile[T * LS + 2];
   ItemRange ir;
   if (!item_range(d, &ir)) return;
   const int tu0 = (ir.tile / d.ntx) * T, tv0 = (ir.tile % d.ntx) * T;
@@ -468,49 +589,91 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
     tile[r * LS + c] = (gu < gsz && gv < gsz) ? grid[(long long)gu * d.pitch + gv]
                                               : make_float2(0.f, 0.f);
   }
+  if (threadIdx.x == 0) tile[DUMMY] = make_float2(0.f, 0.f);
   __syncthreads();
-  const __amdgpu_buffer_rsrc_t rs = tap_rsrc(taps, n_taps);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int base = ir.begin + wave * 64; base < ir.end; base += 256) {
-    const int nb = min(64, ir.end - base);
-    int4 my = make_int4(0, 0, 0, 0);
-    if (lane < nb) my = d.pairs[base + lane];
-    for (int j = 0; j < nb; j++) {
-      Walk k = walk_pair<LS>(my, j, tu0, tv0, T, lane);
-      // P = sum gx*tx, Q = sum gy*ty, R = sum gx*ty, U = sum gy*tx
-      float P = 0.f, Q = 0.f, R = 0.f, U = 0.f;
-      for (int s0 = 0; s0 < k.K; s0 += JOB_U) {
-        float2 t[JOB_U], g[JOB_U];
-#pragma unroll
-        for (int u = 0; u < JOB_U; u++) {
-          const bool act = (s0 + u < k.K) && (k.r < k.h);
-          t[u] = tap_load(rs, act ? (unsigned)k.tb : TAP_OOB);
-          g[u] = tile[act ? k.li : 0];
-          walk_step(k);
+  const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
+  if (npw == 0) return;
+  int4 rec, rec_n;
+  float2 dm, dm_n;
+  stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm);
+  stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n);
+  int k = 0, s0 = 0;
+  Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane);
+  float2 ta[GRP], tb[GRP];
+  load_group(ta, p, 0, rsrc);
+  // P = sum gx*tx, Q = sum gy*ty, R = sum gx*ty, U = sum gy*tx
+  float P = 0.f, Q = 0.f, R = 0.f, U = 0.f;
+  bool more = true;
+  // one group; alternate buffers as in the gridder
+  auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
+    Pair pn = p;
+    int kn = k, sn = s0 + GRP;
+    const bool last = sn >= p.nslabs;
+    if (last) {
+      kn = k + 1;
+      sn = 0;
+      if (kn < npw) {
+        const int j = kn & 63;
+        if (j == 0) {
+          rec = rec_n;
+          stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n);
         }
-#pragma unroll
-        for (int u = 0; u < JOB_U; u++) {
-          P = fmaf(g[u].x, t[u].x, P);
-          Q = fmaf(g[u].y, t[u].y, Q);
-          R = fmaf(g[u].x, t[u].y, R);
-          U = fmaf(g[u].y, t[u].x, U);
-        }
+        pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j),
+                                    lane);
       }
+    }
+    more = kn < npw;
+    load_group(tn, pn, sn, rsrc);
+    __builtin_amdgcn_sched_barrier(0);
+    float2 g[GRP];
+#pragma unroll
+    for (int u = 0; u < GRP; u++) g[u] = tile[group_lds(p, s0 + u, DUMMY)];
+#pragma unroll
+    for (int u = 0; u < GRP; u++) {
+      P = fmaf(g[u].x, tc[u].x, P);
+      Q = fmaf(g[u].y, tc[u].y, Q);
+      R = fmaf(g[u].x, tc[u].y, R);
+      U = fmaf(g[u].y, tc[u].x, U);
+    }
+    if (last) {
       // tap t' = (tx, sg*ty): re = P - sg*Q, im = sg*R + U
-      float re = k.conj ? P + Q : P - Q;
-      float im = k.conj ? U - R : U + R;
+      float re = p.conj ? P + Q : P - Q;
+      float im = p.conj ? U - R : U + R;
       re = wave_sum(re);
       im = wave_sum(im);
       if (lane == 0) {
-        float2 *o = vis + k.point;
-        if (k.whole) {
+        float2 *o = vis + p.point;
+        if (p.whole) {
           *o = make_float2(re, im);
         } else {
           unsafeAtomicAdd(&o->x, re);
           unsafeAtomicAdd(&o->y, im);
         }
       }
+      P = Q = R = U = 0.f;
     }
+    p = pn;
+    k = kn;
+    s0 = sn;
+  };
+  while (true) {
+    body(ta, tb);
+    if (!more) break;
+    body(tb, ta);
+    if (!more) break;
+  }
+  // never true (chunk >= 1): keeps the prefetch loads from sinking
+  if (d.chunk < 0) {
+    float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < GRP; u++) {
+      z.x += ta[u].x + tb[u].x;
+      z.y += ta[u].y + tb[u].y;
+    }
+    vis[lane] = z;
   }
 }
 
