@@ -222,6 +222,43 @@ rcg_status rcg_tiled_stats(rcg_tiled_plan *plan, unsigned long long *taps,
                            long long *n_pairs, long long *n_items, int *err,
                            void *stream);
 
+/* Small kernels on device memory (f64 = 1: complex128/double data, 0:
+ * complex64/float).  Semantics as the host entry points above. */
+rcg_status rcg_normalize(void *grid, int f64, int grid_pitch, int grid_size,
+                         void *stream);
+/* vis *= exp(2 pi i w scale) with the phase and its sine/cosine in FP64. */
+rcg_status rcg_rotate(const rcg_double3 *uvw, void *vis, int f64, int n,
+                      double scale, void *stream);
+rcg_status rcg_grid0(const rcg_double3 *uvw, const rcg_c128 *vis,
+                     rcg_c128 *grid_centred, double scale, int n,
+                     int grid_pitch, int grid_size, void *stream);
+/* count_scratch: grid_size^2 ints of device memory. */
+rcg_status rcg_reweight(const rcg_double3 *uvw, rcg_c128 *vis, double scale,
+                        int n, int grid_size, int *count_scratch, void *stream);
+rcg_status rcg_herm_padded(rcg_c128 *data, int size, int pitch, void *stream);
+rcg_status rcg_fftshift_even(void *data, int elem_bytes, int size, int pitch,
+                             void *stream);
+
+/* rocFFT-backed transform with the semantics of fft_inplace_even (sign -1,
+ * 0, 1, 2; f64 selects double or single precision) on device memory laid
+ * out as the reference's in-place buffer: size rows of pitch complex. */
+typedef struct rcg_fft_plan rcg_fft_plan;
+rcg_status rcg_fft_plan_create(int sign, int f64, int size, int pitch,
+                               rcg_fft_plan **plan);
+rcg_status rcg_fft_execute(rcg_fft_plan *plan, void *data, void *stream);
+rcg_status rcg_fft_plan_destroy(rcg_fft_plan *plan);
+
+/* Hogbom CLEAN on device images (size siz x siz, row pitch `pitch`
+ * elements, double if f64 else float).  scratch: rcg_hogbom_scratch_bytes()
+ * of device memory; rcg_hogbom_iterations reads back how many peaks were
+ * accepted (it synchronises the stream). */
+size_t rcg_hogbom_scratch_bytes(void);
+rcg_status rcg_hogbom(void *model, void *res, const void *psf, int f64, int siz,
+                      int pitch, unsigned niters, double gain, double threshold,
+                      void *scratch, void *stream);
+rcg_status rcg_hogbom_iterations(const void *scratch, unsigned *iters,
+                                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,30 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_degrid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_tiled_stats": (C.c_int, [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_longlong),
                                   C.POINTER(C.c_longlong), C.POINTER(C.c_int), vp]),
+    # small kernels (device)
+    "rcg_normalize": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "rcg_rotate": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, vp]),
+    "rcg_grid0": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
+    "rcg_reweight": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]),
+    "rcg_herm_padded": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+    "rcg_fftshift_even": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "rcg_fft_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+    "rcg_fft_execute": (C.c_int, [vp, vp, vp]),
+    "rcg_fft_plan_destroy": (C.c_int, [vp]),
+    "rcg_hogbom_scratch_bytes": (C.c_size_t, []),
+    "rcg_hogbom": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_double,
+                             C.c_double, vp, vp]),
+    "rcg_hogbom_iterations": (C.c_int, [vp, C.POINTER(C.c_uint), vp]),
+    # reference host entry points next to the gridder
+    "normalizeCPU": (None, [vp, C.c_int, C.c_int]),
+    "rotateCPU": (None, [vp, vp, C.c_int, C.c_double]),
+    "grid0": (None, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int]),
+    "reweight": (None, [vp, vp, C.c_double, C.c_int, C.c_int]),
+    "herm_padded_inplace": (None, [vp, C.c_int, C.c_int]),
+    "fft_inplace_even": (vp, [vp, C.c_int, vp, C.c_int, C.c_int]),
+    "fftInitThreading": (None, []),
+    "rcg_fft_destroy_plan": (None, [vp]),
+    "deconvolve": (None, [vp, vp, vp, C.c_int, C.c_int, C.c_uint, C.c_double, C.c_double]),
 }
 
 

@@ -99,6 +99,13 @@ k_degrid_plain_f64(const rcg_pg *__restrict__ pg,
   }
 }
 
+__global__ void k_zero_f64(double *__restrict__ p, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 static int plain_blocks(long long n)
 {
   // four wavefronts per block, grid-stride beyond 64k blocks
@@ -124,8 +131,11 @@ extern "C" rcg_status rcg_grid_plain_f64(const rcg_geom *geom,
   rcg_status s = check_geom(geom);
   if (s != RCG_OK) return s;
   hipStream_t st = (hipStream_t)stream;
-  const size_t gbytes = (size_t)geom->grid_size * geom->grid_pitch * sizeof(rcg_c128);
-  RCG_HIP_TRY(hipMemsetAsync(grid, 0, gbytes, st));
+  // zeroed by a kernel rather than a memset node, so the call can be
+  // captured into a graph
+  const long long nd = 2LL * geom->grid_size * geom->grid_pitch;
+  hipLaunchKernelGGL(k_zero_f64, dim3(2048), dim3(256), 0, st,
+                     reinterpret_cast<double *>(grid), nd);
   const long long n = (long long)geom->n_chunks * geom->ts_ch;
   if (n == 0) return RCG_OK;
   hipLaunchKernelGGL(k_grid_plain_f64, dim3(plain_blocks(n)), dim3(256), 0, st,

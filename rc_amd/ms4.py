@@ -60,3 +60,81 @@ def degrid(vc: VisChunks, gcf: GCFTable, grid_arr: np.ndarray,
     name = "deGridKernelCPUHalfGCF" if gcf.half else "deGridKernelCPUFullGCF"
     taps = _legacy(name, vc, gcf, g, out, size, pitch)
     return out, taps
+
+
+def _raise_on_error(name: str) -> None:
+    code, msg = _lib.last_error()
+    if code:
+        raise _lib.RcgError(f"{name}: {msg}")
+
+
+def normalize(grid_arr: np.ndarray) -> None:
+    """normalizeCPU in place on a (size, pitch) complex128 grid."""
+    if grid_arr.dtype != np.complex128 or not grid_arr.flags.c_contiguous:
+        raise TypeError("grid must be C-contiguous complex128")
+    size, pitch = grid_arr.shape
+    _lib.lib().normalizeCPU(grid_arr.ctypes.data, pitch, size)
+    _raise_on_error("normalizeCPU")
+
+
+def rotate(uvw: np.ndarray, vis: np.ndarray, scale: float) -> None:
+    """rotateCPU in place: vis *= exp(2 pi i w scale)."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    if vis.dtype != np.complex128 or not vis.flags.c_contiguous:
+        raise TypeError("vis must be C-contiguous complex128")
+    _lib.lib().rotateCPU(uvw.ctypes.data, vis.ctypes.data, vis.size, scale)
+    _raise_on_error("rotateCPU")
+
+
+def grid0(uvw: np.ndarray, vis: np.ndarray, grid_arr: np.ndarray, scale: float) -> None:
+    """grid0: nearest-cell gridding added into grid_arr (size, pitch); the
+    centring is done here as the reference's caller does
+    (MS4/misc/cppcycle.cpp:93)."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex128)
+    size, pitch = grid_arr.shape
+    centre = grid_arr.ctypes.data + (size // 2 * (pitch + 1)) * 16
+    _lib.lib().grid0(uvw.ctypes.data, vis.ctypes.data, centre, scale, vis.size, pitch, size)
+    _raise_on_error("grid0")
+
+
+def reweight(uvw: np.ndarray, vis: np.ndarray, scale: float, grid_size: int) -> None:
+    """reweight in place (uniform weighting)."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    if vis.dtype != np.complex128 or not vis.flags.c_contiguous:
+        raise TypeError("vis must be C-contiguous complex128")
+    _lib.lib().reweight(uvw.ctypes.data, vis.ctypes.data, scale, vis.size, grid_size)
+    _raise_on_error("reweight")
+
+
+def herm_padded_inplace(data: np.ndarray) -> None:
+    size, pitch = data.shape
+    _lib.lib().herm_padded_inplace(data.ctypes.data, size, pitch)
+    _raise_on_error("herm_padded_inplace")
+
+
+def fft_inplace_even(sign: int, data: np.ndarray, size: int, pitch: int) -> None:
+    """fft_inplace_even on a host buffer of size*pitch complex128 (plan
+    created and released per call)."""
+    if data.dtype != np.complex128 or data.size < size * pitch:
+        raise TypeError("data must hold size*pitch complex128")
+    h = _lib.lib()
+    plan = h.fft_inplace_even(None, sign, data.ctypes.data, size, pitch)
+    code, msg = _lib.last_error()
+    if plan:
+        h.rcg_fft_destroy_plan(plan)
+    if code:
+        raise _lib.RcgError(f"fft_inplace_even: {msg}")
+
+
+def deconvolve(res: np.ndarray, psf: np.ndarray, niters: int, gain: float,
+               threshold: float) -> np.ndarray:
+    """deconvolve (Hogbom CLEAN): updates res in place, returns the model."""
+    if res.dtype != np.float64 or psf.dtype != np.float64:
+        raise TypeError("res and psf must be float64")
+    siz, pitch = res.shape
+    model = np.zeros_like(res)
+    _lib.lib().deconvolve(model.ctypes.data, res.ctypes.data, psf.ctypes.data, siz, pitch,
+                          niters, gain, threshold)
+    _raise_on_error("deconvolve")
+    return model

@@ -1,0 +1,44 @@
+"""The C ABI: librcgrid.so loads and exports every function include/rcgrid.h
+declares (no compute call; runs without a GPU)."""
+import ctypes
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = ROOT / "include" / "rcgrid.h"
+LIB = ROOT / "rc_amd" / "librcgrid.so"
+
+MS4_NAMES = {"gridKernelCPUFullGCF", "gridKernelCPUHalfGCF", "deGridKernelCPUFullGCF",
+             "deGridKernelCPUHalfGCF", "grid0", "normalizeCPU", "reweight", "rotateCPU",
+             "herm_padded_inplace", "fft_inplace_even", "fftInitThreading", "deconvolve"}
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    skip = {"if", "defined", "typedef"}
+    return sorted({n for n in names if n not in skip})
+
+
+def test_header_declares_the_reference_entry_points():
+    assert MS4_NAMES <= set(declared_functions())
+
+
+def test_library_exports_every_declared_function():
+    assert LIB.exists(), "build the library first: python -m rc_amd.build"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_library_loads_without_a_gpu():
+    ctypes.CDLL(str(LIB))
+    from rc_amd import _lib
+    h = _lib.lib()
+    assert h.rcg_last_error() == 0
