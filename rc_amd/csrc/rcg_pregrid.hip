@@ -7,6 +7,7 @@
 // every step is a single IEEE FP64 operation with contraction switched off:
 // us = u*scale (one rounding), floor(us), us - floor(us) (exact), 8*(...)
 // (exact), floor; w/wstep (correctly rounded division) and round-half-away.
+#include <algorithm>
 #include <climits>
 
 #include "rcg_internal.h"
@@ -14,16 +15,21 @@
 
 namespace rcg {
 
+// Grid-stride over the points; the in-bound tap count is reduced per
+// wavefront, then per workgroup, and added with one atomic per workgroup
+// (a single counter hit by every wavefront serialises at the L2).
 __global__ void __launch_bounds__(256)
 k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
           unsigned long long *__restrict__ taps_dev, int *__restrict__ err_dev)
 {
+  __shared__ unsigned long long s_taps[4];
+  __shared__ int s_err;
+  if (threadIdx.x == 0) s_err = 0;
   // point counts are below 2^31 (check_geom), so 32-bit index arithmetic
   const int n = g.n_chunks * g.ts_ch;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long my_taps = 0;
   int my_err = 0;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
     PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
                                    t.half != 0);
@@ -41,7 +47,7 @@ k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
         rec.tap0 = tap0;
         rec.stride = (int16_t)stride;
         rec.flags = p.conj ? PG_CONJ : 0;
-        my_taps = (unsigned long long)S * (unsigned long long)S;
+        my_taps += (unsigned long long)S * (unsigned long long)S;
       } else {
         rec.flags = PG_RANGE_ERR;
         my_err = ERR_RANGE;
@@ -49,11 +55,16 @@ k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
     }
     pg[i] = rec;
   }
-  // wavefront reduction of the tap count, one atomic per wavefront
   for (int off = 32; off > 0; off >>= 1) my_taps += __shfl_xor(my_taps, off);
-  if ((threadIdx.x & 63) == 0 && my_taps && taps_dev)
-    atomicAdd(taps_dev, my_taps);
-  if (my_err && err_dev) atomicOr(err_dev, my_err);
+  if ((threadIdx.x & 63) == 0) s_taps[threadIdx.x >> 6] = my_taps;
+  __syncthreads();
+  if (my_err) atomicOr(&s_err, my_err);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long tot = s_taps[0] + s_taps[1] + s_taps[2] + s_taps[3];
+    if (tot && taps_dev) atomicAdd(taps_dev, tot);
+    if (s_err && err_dev) atomicOr(err_dev, s_err);
+  }
 }
 
 // Ranges over the in-bound points: out[0..1] = min/max w-plane, out[2..3] =
@@ -64,17 +75,20 @@ k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
 __global__ void __launch_bounds__(256)
 k_plane_range(rcg_geom g, int half, int *__restrict__ out)
 {
+  __shared__ int s_r[4][4];
   const int n = g.n_chunks * g.ts_ch;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int lo = INT_MAX, hi = INT_MIN, llo = INT_MAX, lhi = INT_MIN;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
     PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
                                    half != 0);
     if (p.inbound) {
-      lo = hi = p.w_plane;
+      lo = min(lo, p.w_plane);
+      hi = max(hi, p.w_plane);
       const int idx = half ? (p.index < 0 ? -p.index : p.index) : p.index;
-      llo = lhi = idx >= 0 ? idx / 64 : -((-idx + 63) / 64);
+      const int lp = idx >= 0 ? idx / 64 : -((-idx + 63) / 64);
+      llo = min(llo, lp);
+      lhi = max(lhi, lp);
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -83,11 +97,27 @@ k_plane_range(rcg_geom g, int half, int *__restrict__ out)
     llo = min(llo, __shfl_xor(llo, off));
     lhi = max(lhi, __shfl_xor(lhi, off));
   }
-  if ((threadIdx.x & 63) == 0 && lo <= hi) {
-    atomicMin(out, lo);
-    atomicMax(out + 1, hi);
-    atomicMin(out + 2, llo);
-    atomicMax(out + 3, lhi);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_r[w][0] = lo;
+    s_r[w][1] = hi;
+    s_r[w][2] = llo;
+    s_r[w][3] = lhi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      lo = min(lo, s_r[k][0]);
+      hi = max(hi, s_r[k][1]);
+      llo = min(llo, s_r[k][2]);
+      lhi = max(lhi, s_r[k][3]);
+    }
+    if (lo <= hi) {
+      atomicMin(out, lo);
+      atomicMax(out + 1, hi);
+      atomicMin(out + 2, llo);
+      atomicMax(out + 3, lhi);
+    }
   }
 }
 
@@ -96,8 +126,8 @@ rcg_status launch_plane_range(const rcg_geom *g, int half, int *out_dev,
 {
   const long long n = (long long)g->n_chunks * g->ts_ch;
   if (n == 0) return RCG_OK;
-  hipLaunchKernelGGL(k_plane_range, dim3(ceil_div(n, 256)), dim3(256), 0,
-                     stream, *g, half, out_dev);
+  const int nb = (int)std::min<long long>(ceil_div(n, 256), 2048);
+  hipLaunchKernelGGL(k_plane_range, dim3(nb), dim3(256), 0, stream, *g, half, out_dev);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
@@ -120,8 +150,9 @@ extern "C" rcg_status rcg_pregrid(const rcg_geom *geom, const rcg_gcf *gcf,
   if (s != RCG_OK) return s;
   const long long n = (long long)geom->n_chunks * geom->ts_ch;
   if (n == 0) return RCG_OK;
-  hipLaunchKernelGGL(k_pregrid, dim3(ceil_div(n, 256)), dim3(256), 0,
-                     (hipStream_t)stream, *geom, *gcf, pg, taps_dev, err_dev);
+  const int nb = (int)std::min<long long>(ceil_div(n, 256), 4096);
+  hipLaunchKernelGGL(k_pregrid, dim3(nb), dim3(256), 0, (hipStream_t)stream, *geom, *gcf,
+                     pg, taps_dev, err_dev);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
