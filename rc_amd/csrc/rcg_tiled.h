@@ -21,6 +21,8 @@ struct TiledDev {
   int *item_start;   // first work item of each tile
   int *totals;       // [0] pairs, [1] items
   int4 *pairs;       // (point, tile) records sorted by tile
+  float2 *partial;   // degridding: per (point, tile slot) partial sums
+  int mpp;           // tile slots per point (max pairs per point)
 };
 
 }  // namespace rcg

@@ -32,10 +32,12 @@ namespace rcg {
 
 // Pair record: one (point, tile) incidence with the geometry of the
 // window/tile intersection precomputed, 16 bytes.
-//   x: r0 | c0 << 6 | (w-1) << 12 | (rows-1) << 18 | conj << 30 | whole << 31
+//   x: r0 | c0 << 6 | (w-1) << 12 | (rows-1) << 18 | slot << 24 | conj << 30
+//      | whole << 31
 //      r0, c0: first tile row/column of the intersection (< 64)
 //      w:      intersection width (1..64)
 //      rows:   64 / w, the rows one 64-lane slab covers
+//      slot:   index of this tile among the point's tiles (row-major)
 //      whole:  the window lies entirely in this tile
 //   y: tap offset (elements) of the intersection's first cell
 //   z: stride | nfull << 16 | hrem << 23
@@ -44,7 +46,7 @@ namespace rcg {
 //      hrem:   rows of the last, partial slab (h % rows)
 //   w: point index
 __device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int point,
-                                          int tile_r, int tile_c, int T)
+                                          int tile_r, int tile_c, int T, int slot)
 {
   const int S = r.supp, stride = r.stride;
   const int tu0 = tile_r * T, tv0 = tile_c * T;
@@ -56,7 +58,8 @@ __device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int p
   const int whole = (h == S) && (w == S);
   int4 p;
   p.x = (ra - tu0) | ((ca - tv0) << 6) | ((w - 1) << 12) | ((rows - 1) << 18) |
-        ((r.flags & PG_CONJ) ? (1 << 30) : 0) | (whole ? (int)0x80000000 : 0);
+        (slot << 24) | ((r.flags & PG_CONJ) ? (1 << 30) : 0) |
+        (whole ? (int)0x80000000 : 0);
   p.y = r.tap0 + (ra - u0) * stride + (ca - v0);
   p.z = stride | (nfull << 16) | (hrem << 23);
   p.w = point;
@@ -232,7 +235,7 @@ k_bin_fill(const rcg_pg *__restrict__ pg, long long n, int pitch, int lt,
         const long long pos =
             (long long)hist[a * ntx + b] +
             rank[((k * BIN_THREADS) + threadIdx.x) * max_pairs_pt + j];
-        if (pos < pair_cap) pairs[pos] = make_pair(u0, v0, r, (int)i, a, b, 1 << lt);
+        if (pos < pair_cap) pairs[pos] = make_pair(u0, v0, r, (int)i, a, b, 1 << lt, j);
         else atomicOr(err, ERR_CAPACITY);
       }
   }
@@ -340,7 +343,7 @@ __device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned vo
 constexpr int GRP = 4;
 
 struct Pair {
-  int h, rows, nslabs, point, whole, conj;
+  int h, rows, nslabs, point, whole, conj, slot;
   int lr;          // lane row (idle lanes: 64)
   unsigned tb0;    // tap byte offset of the lane's slab-0 cell
   int li0;         // LDS index of the lane's slab-0 cell
@@ -363,6 +366,7 @@ __device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int 
   p.nslabs = nfull + (hrem ? 1 : 0);
   p.conj = (px >> 30) & 1;
   p.whole = px < 0;
+  p.slot = (px >> 24) & 63;
   // lane / w through the approximate reciprocal: (lane + 0.5) / w is at
   // least 0.5 / w away from an integer, far above its rounding error
   const int lr = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)w));
@@ -644,14 +648,12 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       float im = p.conj ? U - R : U + R;
       re = wave_sum(re);
       im = wave_sum(im);
+      // a point inside one tile is final; a split point leaves its part in
+      // its slot for k_degrid_reduce
       if (lane == 0) {
-        float2 *o = vis + p.point;
-        if (p.whole) {
-          *o = make_float2(re, im);
-        } else {
-          unsafeAtomicAdd(&o->x, re);
-          unsafeAtomicAdd(&o->y, im);
-        }
+        float2 *o = p.whole ? vis + p.point
+                            : d.partial + (long long)p.point * d.mpp + p.slot;
+        *o = make_float2(re, im);
       }
       P = Q = R = U = 0.f;
     }
@@ -675,6 +677,30 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
     }
     vis[lane] = z;
   }
+}
+
+// Split points: the sum of their per-tile parts in slot order (fixed order,
+// so the FP32 result is reproducible).  Out-of-bound points keep the zero
+// the entry point wrote.
+__global__ void __launch_bounds__(256)
+k_degrid_reduce(TiledDev d, const rcg_pg *__restrict__ pg, long long n,
+                float2 *__restrict__ vis)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const rcg_pg r = pg[i];
+  if (r.cell < 0 || r.supp <= 0) return;
+  int u0, v0, tu0, tu1, tv0, tv1;
+  point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+  const int np = (tu1 - tu0 + 1) * (tv1 - tv0 + 1);
+  if (np <= 1 || np > d.mpp) return;  // whole (written by the tile kernel)
+  const float2 *q = d.partial + i * d.mpp;
+  float2 s = q[0];
+  for (int k = 1; k < np; k++) {
+    s.x += q[k].x;
+    s.y += q[k].y;
+  }
+  vis[i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -743,6 +769,12 @@ extern "C" rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile,
   }
   const int per_dim = (max_support - 1 + tile - 1) / tile + 1;
   const int max_pairs_pt = per_dim * per_dim;
+  if (max_pairs_pt > 64) {  // the pair record holds the tile slot in 6 bits
+    set_error(RCG_ERR_UNSUPPORTED,
+              "max_support %d spans %d tiles of %d; at most 64 are supported",
+              max_support, max_pairs_pt, tile);
+    return RCG_ERR_UNSUPPORTED;
+  }
   const long long n = (long long)geom->n_chunks * geom->ts_ch;
   rcg_tiled_plan *pl = new (std::nothrow) rcg_tiled_plan();
   if (!pl) {
@@ -777,6 +809,8 @@ extern "C" rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile,
   if (e == hipSuccess) e = dalloc(&pl->taps, 1);
   if (e == hipSuccess) e = dalloc(&pl->err, 1);
   if (e == hipSuccess) e = dalloc(&d.pairs, (size_t)d.pair_cap);
+  d.mpp = max_pairs_pt;
+  if (e == hipSuccess) e = dalloc(&d.partial, (size_t)n * max_pairs_pt);
   if (e != hipSuccess) {
     rcg_tiled_plan_destroy(pl);
     set_error(RCG_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
@@ -791,7 +825,7 @@ extern "C" rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *pl)
   if (!pl) return RCG_OK;
   TiledDev &d = pl->dev;
   void *ptrs[] = {pl->pg, d.tile_count, d.tile_start, d.tile_cursor,
-                  d.item_start, d.totals, pl->taps, pl->err, d.pairs};
+                  d.item_start, d.totals, pl->taps, pl->err, d.pairs, d.partial};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete pl;
@@ -920,6 +954,9 @@ extern "C" rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gc
       set_error(RCG_ERR_UNSUPPORTED, "tile %d not supported by the FP32 degridder", d.T);
       return RCG_ERR_UNSUPPORTED;
   }
+  RCG_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_degrid_reduce, dim3((unsigned)((pl->n + 255) / 256)), dim3(256), 0, st,
+                     d, pl->pg, pl->n, v);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
