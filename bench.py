@@ -91,8 +91,8 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the path "
-                          "shards across GPUs by design (see DESIGN.md), but no "
-                          "multi-GPU measurement is made", "n_gpus": args.gpus}))
+                          "is designed to shard across GPUs, but no multi-GPU "
+                          "measurement is made", "n_gpus": args.gpus}))
         return 2
     import torch
     from rc_amd import device as D
@@ -134,19 +134,24 @@ def main() -> int:
         print(json.dumps({"error": f"binning error word {st['err']}"}))
         return 1
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    plan.timing(True)  # HIP events around the grid and degrid kernels themselves
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step32(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    k_grid = float(np.mean(plan.kernel_times("grid")))
+    k_degrid = float(np.mean(plan.kernel_times("degrid")))
+    plan.timing(False)
     ms_step = wall / args.steps * 1e3
     t_bin = np.mean([e[0].elapsed_time(e[1]) for e in evs])
     t_grid = np.mean([e[1].elapsed_time(e[2]) for e in evs])
     t_degrid = np.mean([e[2].elapsed_time(e[3]) for e in evs])
     value = vc.n / (ms_step / 1e3) / 1e6
     log(f"fp32 tiled: {ms_step:.3f} ms/step (bin {t_bin:.3f} grid {t_grid:.3f} "
-        f"degrid {t_degrid:.3f} ms) -> {value:.1f} Mvis/s; stats {st}")
+        f"degrid {t_degrid:.3f} ms; kernels grid {k_grid:.3f} degrid {k_degrid:.3f} ms) "
+        f"-> {value:.1f} Mvis/s; stats {st}")
 
     extra = {}
     taps = st["taps"]
@@ -187,12 +192,14 @@ def main() -> int:
                  "error_vs_fp64": {"grid_rel_l2": g_err, "degrid_rel_l2": d_err}}
 
     # roofline of the dominant kernel (algorithmic bytes per launch: points x
-    # (24 B uvw + 8 B vis) + 8 B per tap + 8 B per grid cell)
+    # (24 B uvw + 8 B vis) + 8 B per tap + 8 B per grid cell), timed by the
+    # HIP events the library records around that kernel on its stream
     b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
-    t_dom = max(t_grid, t_degrid)
-    dom = "grid" if t_grid >= t_degrid else "degrid"
+    t_dom = max(k_grid, k_degrid)
+    dom = "k_grid_tiles" if k_grid >= k_degrid else "k_degrid_tiles"
     achieved = b_dir / (t_dom / 1e3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+    roof = {"bound": "hbm", "kernel": f"{dom}<{args.tile}>", "kernel_ms": round(t_dom, 4),
+            "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "valu_tflops": round(8 * taps / (t_dom / 1e3) / 1e12, 3),

@@ -196,20 +196,37 @@ rcg_status rcg_degrid_plain_f64(const rcg_geom *geom, const rcg_gcf *gcf,
                                 const rcg_pg *pg, const rcg_c128 *grid,
                                 rcg_c128 *vis, void *stream);
 
-/* Tiled FP32 path.  A plan owns the device scratch for one geometry (point
- * count and grid shape): pregrid records, per-tile bins of (point, tile)
- * pairs and work-item tables.  tile = 32 or 64 grid cells; chunk = maximum
- * pairs per work item; max_support bounds every chunk support.
- * rcg_tiled_bin pregrids and bins the points (device only, no host sync);
+/* Tiled FP32 path.  A plan owns the device scratch for one or more
+ * instances (snapshots, facets) of one grid shape: pregrid records, per-tile
+ * bins of (point, tile) pairs, work-item tables and queues.  tile = 16, 32
+ * or 64 grid cells; chunk = maximum pairs per work item; max_support bounds
+ * every chunk support.
+ *
+ * rcg_tiled_plan_create_batch takes n_inst geometries (copied into the
+ * plan: their device buffers must stay valid while the plan is used) with
+ * equal grid_size and grid_pitch; their points are concatenated in
+ * instance order in vis and vis_out, and grid holds n_inst grids of
+ * grid_size x grid_pitch one after the other.  All instances share the GCF
+ * table.  rcg_tiled_plan_create is the one-instance case.
+ *
+ * rcg_tiled_bin pregrids and bins every point (device only, no host sync);
  * grid/degrid then run on complex64 visibilities, taps (gcf->data) and grid,
- * accumulating in FP32.  Every call is asynchronous on `stream`; calls that
- * share a plan must be ordered on one stream.  No call allocates. */
+ * accumulating in FP32, each as one launch over all instances with a fixed
+ * number of workgroups drawing work items from per-XCD queues.  Every call
+ * is asynchronous on `stream`; calls that share a plan must be ordered on
+ * one stream.  None of them allocates memory, synchronises or copies from
+ * the host, so a sequence of them can be captured in a graph.  These entry
+ * points have no counterpart in the reference, whose gridder runs on host
+ * memory (MS4/kernel/common/scatter_gridder_w_dependent_dyn_1p.h:16-91). */
 typedef struct rcg_tiled_plan rcg_tiled_plan;
 rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile, int chunk,
                                  int max_support, rcg_tiled_plan **plan);
+rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_inst,
+                                       int tile, int chunk, int max_support,
+                                       rcg_tiled_plan **plan);
 rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *plan);
-rcg_status rcg_tiled_bin(rcg_tiled_plan *plan, const rcg_geom *geom,
-                         const rcg_gcf *gcf, void *stream);
+rcg_status rcg_tiled_bin(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                         void *stream);
 rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                               const rcg_c64 *vis, rcg_c64 *grid, void *stream);
 rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
@@ -221,6 +238,29 @@ rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
 rcg_status rcg_tiled_stats(rcg_tiled_plan *plan, unsigned long long *taps,
                            long long *n_pairs, long long *n_items, int *err,
                            void *stream);
+/* Kernel timing: with timing on, every grid (which = 0) or degrid
+ * (which = 1) call records HIP events around its main kernel on its
+ * stream; rcg_tiled_kernel_times waits for and returns the durations (ms)
+ * of the last min(count, 64, max) such launches, oldest first, and clears
+ * the record.  Switching timing on or off clears it too. */
+rcg_status rcg_tiled_timing(rcg_tiled_plan *plan, int enable);
+rcg_status rcg_tiled_kernel_times(rcg_tiled_plan *plan, int which, float *ms,
+                                  int max, int *count);
+
+/* Graphs: rcg_tiled_graph_create captures the chosen stages (bitwise OR of
+ * RCG_STAGE_*) of one plan on fixed buffers into an executable HIP graph;
+ * rcg_graph_launch replays it on `stream` (the buffers' current contents
+ * are read on every replay).  The graph holds no memset nodes. */
+#define RCG_STAGE_BIN 1
+#define RCG_STAGE_GRID 2
+#define RCG_STAGE_DEGRID 4
+typedef struct rcg_graph rcg_graph;
+rcg_status rcg_tiled_graph_create(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                                  const rcg_c64 *vis, rcg_c64 *grid,
+                                  rcg_c64 *vis_out, int stages,
+                                  rcg_graph **graph);
+rcg_status rcg_graph_launch(rcg_graph *graph, void *stream);
+rcg_status rcg_graph_destroy(rcg_graph *graph);
 
 /* Small kernels on device memory (f64 = 1: complex128/double data, 0:
  * complex64/float).  Semantics as the host entry points above. */

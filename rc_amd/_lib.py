@@ -68,8 +68,17 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_degrid_plain_f64": (C.c_int, [C.POINTER(Geom), C.POINTER(Gcf), vp, vp, vp, vp]),
     "rcg_tiled_plan_create": (C.c_int, [C.POINTER(Geom), C.c_int, C.c_int, C.c_int,
                                         C.POINTER(vp)]),
+    "rcg_tiled_plan_create_batch": (C.c_int, [C.POINTER(Geom), C.c_int, C.c_int, C.c_int,
+                                              C.c_int, C.POINTER(vp)]),
     "rcg_tiled_plan_destroy": (C.c_int, [vp]),
-    "rcg_tiled_bin": (C.c_int, [vp, C.POINTER(Geom), C.POINTER(Gcf), vp]),
+    "rcg_tiled_bin": (C.c_int, [vp, C.POINTER(Gcf), vp]),
+    "rcg_tiled_timing": (C.c_int, [vp, C.c_int]),
+    "rcg_tiled_kernel_times": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int,
+                                         C.POINTER(C.c_int)]),
+    "rcg_tiled_graph_create": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp, C.c_int,
+                                         C.POINTER(vp)]),
+    "rcg_graph_launch": (C.c_int, [vp, vp]),
+    "rcg_graph_destroy": (C.c_int, [vp]),
     "rcg_grid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_degrid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_tiled_stats": (C.c_int, [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_longlong),

@@ -49,4 +49,11 @@ rcg_status check_gcf(const rcg_gcf *t);
 rcg_status launch_plane_range(const rcg_geom *g, int half, int *out_dev,
                               hipStream_t stream);
 
+// Pregrid of n_inst geometries (device array) whose points are concatenated:
+// instance k owns points [pt_off[k], pt_off[k + 1]) of pg, n = pt_off[n_inst].
+rcg_status launch_pregrid_batch(const rcg_geom *geoms_dev, const int *pt_off_dev, int n_inst,
+                                long long n, const rcg_gcf *gcf, rcg_pg *pg,
+                                unsigned long long *taps_dev, int *err_dev,
+                                hipStream_t stream);
+
 }  // namespace rcg

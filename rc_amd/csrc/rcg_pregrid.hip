@@ -15,46 +15,45 @@
 
 namespace rcg {
 
-// Grid-stride over the points; the in-bound tap count is reduced per
-// wavefront, then per workgroup, and added with one atomic per workgroup
-// (a single counter hit by every wavefront serialises at the L2).
-__global__ void __launch_bounds__(256)
-k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
-          unsigned long long *__restrict__ taps_dev, int *__restrict__ err_dev)
+// One point: its record, plus the in-bound taps and error bits it adds.
+__device__ __forceinline__ void pregrid_record(const rcg_geom &g, const rcg_gcf &t, int i,
+                                               rcg_pg *__restrict__ out,
+                                               unsigned long long &taps, int &err)
+{
+  const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
+  PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size, t.half != 0);
+  rcg_pg rec;
+  rec.cell = -1;
+  rec.tap0 = 0;
+  rec.stride = 0;
+  rec.supp = (int16_t)S;
+  rec.flags = 0;
+  if (p.inbound) {
+    int tap0 = 0, stride = 0;
+    bool ok = resolve_window(t, p, S, &tap0, &stride);
+    if (ok) {
+      rec.cell = p.u * g.grid_pitch + p.v;
+      rec.tap0 = tap0;
+      rec.stride = (int16_t)stride;
+      rec.flags = p.conj ? PG_CONJ : 0;
+      taps += (unsigned long long)S * (unsigned long long)S;
+    } else {
+      rec.flags = PG_RANGE_ERR;
+      err = ERR_RANGE;
+    }
+  }
+  *out = rec;
+}
+
+// Per-workgroup reduction of the tap count and error bits, then one atomic
+// each (a single counter hit by every wavefront serialises at the L2).
+__device__ __forceinline__ void pregrid_flush(unsigned long long my_taps, int my_err,
+                                              unsigned long long *__restrict__ taps_dev,
+                                              int *__restrict__ err_dev)
 {
   __shared__ unsigned long long s_taps[4];
   __shared__ int s_err;
   if (threadIdx.x == 0) s_err = 0;
-  // point counts are below 2^31 (check_geom), so 32-bit index arithmetic
-  const int n = g.n_chunks * g.ts_ch;
-  unsigned long long my_taps = 0;
-  int my_err = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
-    PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size,
-                                   t.half != 0);
-    rcg_pg rec;
-    rec.cell = -1;
-    rec.tap0 = 0;
-    rec.stride = 0;
-    rec.supp = (int16_t)S;
-    rec.flags = 0;
-    if (p.inbound) {
-      int tap0 = 0, stride = 0;
-      bool ok = resolve_window(t, p, S, &tap0, &stride);
-      if (ok) {
-        rec.cell = p.u * g.grid_pitch + p.v;
-        rec.tap0 = tap0;
-        rec.stride = (int16_t)stride;
-        rec.flags = p.conj ? PG_CONJ : 0;
-        my_taps += (unsigned long long)S * (unsigned long long)S;
-      } else {
-        rec.flags = PG_RANGE_ERR;
-        my_err = ERR_RANGE;
-      }
-    }
-    pg[i] = rec;
-  }
   for (int off = 32; off > 0; off >>= 1) my_taps += __shfl_xor(my_taps, off);
   if ((threadIdx.x & 63) == 0) s_taps[threadIdx.x >> 6] = my_taps;
   __syncthreads();
@@ -65,6 +64,56 @@ k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
     if (tot && taps_dev) atomicAdd(taps_dev, tot);
     if (s_err && err_dev) atomicOr(err_dev, s_err);
   }
+}
+
+// Grid-stride over the points of one geometry (point counts are below 2^31,
+// check_geom, so 32-bit index arithmetic).
+__global__ void __launch_bounds__(256)
+k_pregrid(rcg_geom g, rcg_gcf t, rcg_pg *__restrict__ pg,
+          unsigned long long *__restrict__ taps_dev, int *__restrict__ err_dev)
+{
+  const int n = g.n_chunks * g.ts_ch;
+  unsigned long long my_taps = 0;
+  int my_err = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    pregrid_record(g, t, i, pg + i, my_taps, my_err);
+  pregrid_flush(my_taps, my_err, taps_dev, err_dev);
+}
+
+// Grid-stride over the concatenated points of n_inst geometries; point i
+// belongs to the instance k with pt_off[k] <= i < pt_off[k + 1].
+__global__ void __launch_bounds__(256)
+k_pregrid_batch(const rcg_geom *__restrict__ geoms, const int *__restrict__ pt_off,
+                int n_inst, rcg_gcf t, rcg_pg *__restrict__ pg,
+                unsigned long long *__restrict__ taps_dev, int *__restrict__ err_dev)
+{
+  const int n = pt_off[n_inst];
+  unsigned long long my_taps = 0;
+  int my_err = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int lo = 0, hi = n_inst - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pt_off[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const rcg_geom g = geoms[lo];
+    pregrid_record(g, t, i - pt_off[lo], pg + i, my_taps, my_err);
+  }
+  pregrid_flush(my_taps, my_err, taps_dev, err_dev);
+}
+
+rcg_status launch_pregrid_batch(const rcg_geom *geoms_dev, const int *pt_off_dev, int n_inst,
+                                long long n, const rcg_gcf *gcf, rcg_pg *pg,
+                                unsigned long long *taps_dev, int *err_dev,
+                                hipStream_t stream)
+{
+  if (n == 0) return RCG_OK;
+  const int nb = (int)std::min<long long>(ceil_div(n, 256), 4096);
+  hipLaunchKernelGGL(k_pregrid_batch, dim3(nb), dim3(256), 0, stream, geoms_dev, pt_off_dev,
+                     n_inst, *gcf, pg, taps_dev, err_dev);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
 }
 
 // Ranges over the in-bound points: out[0..1] = min/max w-plane, out[2..3] =

@@ -1,39 +1,68 @@
 // rcg_tiled.h -- device-side state of a tiled FP32 gridding plan.
 #pragma once
 
+#include <vector>
+
 #include "rcg_internal.h"
 
 namespace rcg {
 
-// Tiles per grid side^2 handled by the LDS histograms of the binning pass
+// Tiles per instance handled by the LDS histograms of the binning pass
 // (16384 ints = 64 KiB of LDS).
 constexpr int MAX_BIN_TILES = 16384;
+// Tiles of a whole plan (instances x tiles per instance) handled by the
+// three-kernel scan: 1024 segments of 1024 tiles.
+constexpr int MAX_PLAN_TILES = 1 << 20;
 // Error word bit: a pair did not fit the plan's pair buffer (a chunk
 // support above the plan's max_support).
 constexpr int ERR_CAPACITY = 2;
+// Work-queue counters: [0, 8) gridding, [8, 16) degridding, one per XCD.
+constexpr int N_QUEUE = 16;
 
 struct TiledDev {
-  int T, lt, ntx, n_tiles, chunk, pitch, grid_size;
+  int T, lt, ntx;
+  int tpi;           // tiles per instance (ntx * ntx)
+  int n_inst;
+  int n_tiles;       // n_inst * tpi
+  int chunk, pitch, grid_size;
+  int mpp;           // tile slots per point (max pairs per point)
+  int ppt;           // points per binning thread
+  long long gstride; // grid elements per instance (grid_size * pitch)
   long long pair_cap;
   int *tile_count;   // pairs per tile
   int *tile_start;   // first pair of each tile
   int *tile_cursor;  // fill cursors
   int *item_start;   // first work item of each tile
   int *totals;       // [0] pairs, [1] items
+  int *seg;          // scan scratch: 2 ints per segment of 1024 tiles
+  int *queue;        // N_QUEUE work counters
+  const int *pt_off;   // n_inst + 1: first point of each instance
+  const int *blk_off;  // n_inst + 1: first binning workgroup of each instance
   int4 *pairs;       // (point, tile) records sorted by tile
   float2 *partial;   // degridding: per (point, tile slot) partial sums
-  int mpp;           // tile slots per point (max pairs per point)
 };
+
+// Event pairs kept per direction for rcg_tiled_kernel_times.
+constexpr int N_TIMED = 64;
 
 }  // namespace rcg
 
 struct rcg_tiled_plan {
-  rcg_geom geom;
-  long long n;
-  int max_support;
-  int max_pairs_pt;
+  std::vector<rcg_geom> geoms;
+  rcg_geom *geoms_dev = nullptr;
+  long long n = 0;
+  int max_support = 0;
+  int max_pairs_pt = 0;
+  int grid_blocks = 0;    // persistent workgroups of the grid kernel
+  int degrid_blocks = 0;  // ... of the degrid kernel
+  int bin_blocks = 0;
   rcg_pg *pg = nullptr;
   unsigned long long *taps = nullptr;
   int *err = nullptr;
+  int *offs = nullptr;    // device pt_off | blk_off
   rcg::TiledDev dev{};
+  // kernel timing (rcg_tiled_timing)
+  bool timing = false;
+  hipEvent_t ev[2][rcg::N_TIMED][2] = {};
+  int n_ev[2] = {0, 0};
 };

@@ -2,27 +2,33 @@
 //
 // Same mathematics as the reference's scatter gridder and its transpose
 // degridder (MS4/kernel/cpu/scatter_grid/scatter_gridder_w_dependent_dyn_1p
-// .cpp:146-176), organised for CDNA4:
+// .cpp:146-176), organised for CDNA4.  A plan covers one or more instances
+// (snapshots, facets): each has its own points and geometry and its own
+// grid of the common shape; all share one GCF table.
 //
 //  1. pregrid (FP64, exact; rcg_pregrid.hip) -> one record per point;
-//  2. binning: the grid is cut into T x T tiles; every in-bound point is
+//  2. binning: every grid is cut into T x T tiles; every in-bound point is
 //     listed in each tile its S x S window overlaps (a point near a tile
 //     edge appears in up to four tiles when S <= T).  A counting sort builds
 //     the per-tile lists without host synchronisation: LDS histograms per
-//     workgroup -> global counts -> one-workgroup scan -> LDS-ranked fill;
+//     workgroup (each workgroup holds points of one instance) -> global
+//     counts -> three-kernel scan -> LDS-ranked fill;
 //  3. work items: each tile's list is cut into chunks of at most `chunk`
-//     pairs; workgroup b takes item b (XCD-aware order) and exits when b is
-//     past the device-side item count, so the launch size is an upper
-//     bound known on the host and nothing waits on the host;
-//  4. gridding: the workgroup accumulates its items' taps into an FP32 tile
-//     in LDS (ds_add_f32), lanes of a wavefront covering the rows and
-//     columns of the window/tile intersection, GCF rows read coalesced;
-//     the tile is stored to HBM with plain stores when the tile has a single
-//     item, with FP32 atomics (non-zero cells only) otherwise;
+//     pairs.  The grid and degrid kernels are persistent: a fixed number of
+//     workgroups draw items from eight queues, one per XCD, each holding a
+//     contiguous range of items (neighbouring tiles share that XCD's L2),
+//     and steal from the other queues when theirs is empty;
+//  4. gridding: each wavefront accumulates its pairs' taps into a private
+//     FP32 copy of the tile in LDS, lanes covering the rows and columns of
+//     the window/tile intersection, GCF rows read coalesced; the four copies
+//     are summed and stored to HBM with plain stores when the tile has a
+//     single item, with FP32 atomics (non-zero cells only) otherwise;
 //  5. degridding: the workgroup stages its grid tile in LDS, each wavefront
 //     forms the partial sum of one pair over the intersection and reduces it
 //     across lanes; points whose window lies in one tile are stored, split
-//     points are added atomically.
+//     points leave one partial per tile, summed in a fixed order by
+//     k_degrid_reduce.
+#include <algorithm>
 #include <climits>
 
 #include "rcg_internal.h"
@@ -88,12 +94,13 @@ __global__ void k_fill_f32(float *__restrict__ p, long long n)
 // ---------------------------------------------------------------------------
 // binning
 //
-// Each workgroup (BIN_THREADS threads) takes BIN_PPT consecutive points per
-// thread.  Tile ranges of a point: rows [u0 >> lt, (u0 + S - 1) >> lt],
-// columns likewise.
+// Each workgroup (BIN_THREADS threads) takes ppt consecutive points per
+// thread, all of one instance: instance k owns workgroups
+// [blk_off[k], blk_off[k + 1]).  Tile ranges of a point: rows
+// [u0 >> lt, (u0 + S - 1) >> lt], columns likewise; plan tile index =
+// instance * tpi + row * ntx + column.
 
 constexpr int BIN_THREADS = 1024;
-constexpr int BIN_PPT = 4;
 
 __device__ __forceinline__ void point_tiles(const rcg_pg &r, int pitch, int lt,
                                             int *u0, int *v0, int *tu0,
@@ -108,146 +115,198 @@ __device__ __forceinline__ void point_tiles(const rcg_pg &r, int pitch, int lt,
   *tv1 = (*v0 + S - 1) >> lt;
 }
 
+struct BinBlock {
+  int inst, p0, p1;  // points [p0, p1) of instance inst
+};
+
+__device__ __forceinline__ BinBlock bin_block(const TiledDev &d)
+{
+  // the largest k with blk_off[k] <= blockIdx.x (instances without points
+  // own no workgroup and are skipped)
+  int lo = 0, hi = d.n_inst - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.blk_off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  BinBlock r;
+  r.inst = lo;
+  r.p0 = d.pt_off[lo] + (b - d.blk_off[lo]) * (BIN_THREADS * d.ppt);
+  r.p1 = min(d.pt_off[lo + 1], r.p0 + BIN_THREADS * d.ppt);
+  return r;
+}
+
 __global__ void __launch_bounds__(BIN_THREADS)
-k_bin_count(const rcg_pg *__restrict__ pg, long long n, int pitch, int lt,
-            int ntx, int n_tiles, int max_pairs_pt, int *__restrict__ tile_count,
-            int *__restrict__ err)
+k_bin_count(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 {
   extern __shared__ int hist[];
-  for (int t = threadIdx.x; t < n_tiles; t += BIN_THREADS) hist[t] = 0;
+  for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) hist[t] = 0;
   __syncthreads();
-  const long long base = (long long)blockIdx.x * BIN_THREADS * BIN_PPT;
-  for (int k = 0; k < BIN_PPT; k++) {
-    const long long i = base + (long long)k * BIN_THREADS + threadIdx.x;
-    if (i >= n) break;
+  const BinBlock bb = bin_block(d);
+  for (int k = 0; k < d.ppt; k++) {
+    const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
+    if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
     if (r.cell < 0 || r.supp <= 0) continue;
     int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, pitch, lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > max_pairs_pt) {
+    point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) {
       atomicOr(err, ERR_CAPACITY);  // support above the plan's max_support
       continue;
     }
     for (int a = tu0; a <= tu1; a++)
-      for (int b = tv0; b <= tv1; b++) atomicAdd(&hist[a * ntx + b], 1);
+      for (int b = tv0; b <= tv1; b++) atomicAdd(&hist[a * d.ntx + b], 1);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < n_tiles; t += BIN_THREADS)
-    if (hist[t]) atomicAdd(&tile_count[t], hist[t]);
+  int *tc = d.tile_count + (long long)bb.inst * d.tpi;
+  for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS)
+    if (hist[t]) atomicAdd(&tc[t], hist[t]);
 }
 
-// One workgroup: exclusive scans of the tile counts (pairs) and of the
-// per-tile item counts; totals[0] = pairs, totals[1] = items; clears the
-// fill cursors.
-constexpr int SCAN_THREADS = 1024;
+// Scan of the tile counts (pairs) and per-tile item counts in three
+// kernels over segments of SCAN_SEG tiles: segment sums, one workgroup
+// scanning the sums, per-segment exclusive scans.  totals[0] = pairs,
+// totals[1] = items; the fill cursors are cleared.
+constexpr int SCAN_SEG = 1024;
 
-__global__ void __launch_bounds__(SCAN_THREADS)
-k_bin_scan(const int *__restrict__ tile_count, int n_tiles, int chunk,
-           int *__restrict__ tile_start, int *__restrict__ item_start,
-           int *__restrict__ tile_cursor, int *__restrict__ totals)
+// Exclusive scan of (x, y) over a 1024-thread workgroup; *tot = the sums.
+__device__ __forceinline__ int2 block_scan(int2 v, int2 *sh, int2 *tot)
 {
-  __shared__ long long s_pairs[SCAN_THREADS];
-  __shared__ long long s_items[SCAN_THREADS];
-  const int per = (n_tiles + SCAN_THREADS - 1) / SCAN_THREADS;
-  const int t0 = threadIdx.x * per;
-  long long lp = 0, li = 0;
-  for (int t = t0; t < t0 + per && t < n_tiles; t++) {
-    const int c = tile_count[t];
-    lp += c;
-    li += (c + chunk - 1) / chunk;
-  }
-  s_pairs[threadIdx.x] = lp;
-  s_items[threadIdx.x] = li;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 partial sums
-  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-    long long ap = 0, ai = 0;
-    if (threadIdx.x >= off) {
-      ap = s_pairs[threadIdx.x - off];
-      ai = s_items[threadIdx.x - off];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int2 x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int ax = __shfl_up(x.x, off), ay = __shfl_up(x.y, off);
+    if (lane >= off) {
+      x.x += ax;
+      x.y += ay;
     }
-    __syncthreads();
-    s_pairs[threadIdx.x] += ap;
-    s_items[threadIdx.x] += ai;
-    __syncthreads();
   }
-  long long rp = s_pairs[threadIdx.x] - lp;
-  long long ri = s_items[threadIdx.x] - li;
-  for (int t = t0; t < t0 + per && t < n_tiles; t++) {
-    const int c = tile_count[t];
-    tile_start[t] = (int)rp;
-    item_start[t] = (int)ri;
-    tile_cursor[t] = 0;
-    rp += c;
-    ri += (c + chunk - 1) / chunk;
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    int2 w = lane < 16 ? sh[lane] : make_int2(0, 0);
+    for (int off = 1; off < 16; off <<= 1) {
+      const int ax = __shfl_up(w.x, off), ay = __shfl_up(w.y, off);
+      if (lane >= off) {
+        w.x += ax;
+        w.y += ay;
+      }
+    }
+    if (lane < 16) sh[lane] = w;
   }
-  if (threadIdx.x == SCAN_THREADS - 1) {
-    totals[0] = (int)s_pairs[SCAN_THREADS - 1];
-    totals[1] = (int)s_items[SCAN_THREADS - 1];
+  __syncthreads();
+  const int2 base = wave ? sh[wave - 1] : make_int2(0, 0);
+  *tot = sh[15];
+  return make_int2(base.x + x.x - v.x, base.y + x.y - v.y);
+}
+
+__global__ void __launch_bounds__(SCAN_SEG)
+k_scan_reduce(TiledDev d)
+{
+  __shared__ int2 sh[16];
+  const int t = blockIdx.x * SCAN_SEG + threadIdx.x;
+  const int c = t < d.n_tiles ? d.tile_count[t] : 0;
+  int2 tot;
+  block_scan(make_int2(c, (c + d.chunk - 1) / d.chunk), sh, &tot);
+  if (threadIdx.x == 0) {
+    d.seg[2 * blockIdx.x] = tot.x;
+    d.seg[2 * blockIdx.x + 1] = tot.y;
+  }
+}
+
+__global__ void __launch_bounds__(SCAN_SEG)
+k_scan_top(TiledDev d, int n_seg)
+{
+  __shared__ int2 sh[16];
+  const int s = threadIdx.x;
+  const int2 v = s < n_seg ? make_int2(d.seg[2 * s], d.seg[2 * s + 1]) : make_int2(0, 0);
+  int2 tot;
+  const int2 e = block_scan(v, sh, &tot);
+  if (s < n_seg) {
+    d.seg[2 * s] = e.x;
+    d.seg[2 * s + 1] = e.y;
+  }
+  if (s == 0) {
+    d.totals[0] = tot.x;
+    d.totals[1] = tot.y;
+  }
+}
+
+__global__ void __launch_bounds__(SCAN_SEG)
+k_scan_apply(TiledDev d)
+{
+  __shared__ int2 sh[16];
+  const int t = blockIdx.x * SCAN_SEG + threadIdx.x;
+  const int c = t < d.n_tiles ? d.tile_count[t] : 0;
+  int2 tot;
+  const int2 e = block_scan(make_int2(c, (c + d.chunk - 1) / d.chunk), sh, &tot);
+  if (t < d.n_tiles) {
+    d.tile_start[t] = d.seg[2 * blockIdx.x] + e.x;
+    d.item_start[t] = d.seg[2 * blockIdx.x + 1] + e.y;
+    d.tile_cursor[t] = 0;
   }
 }
 
 // Fill: LDS ranks within the workgroup, one global atomic per touched tile
 // for the workgroup's base, then every pair is written to its slot.
 __global__ void __launch_bounds__(BIN_THREADS)
-k_bin_fill(const rcg_pg *__restrict__ pg, long long n, int pitch, int lt,
-           int ntx, int n_tiles, int max_pairs_pt,
-           const int *__restrict__ tile_start, int *__restrict__ tile_cursor,
-           int4 *__restrict__ pairs, long long pair_cap, int *__restrict__ err)
+k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 {
   extern __shared__ int lds[];
-  int *hist = lds;                                        // n_tiles
-  uint16_t *rank = reinterpret_cast<uint16_t *>(lds + n_tiles);  // per pair
-  for (int t = threadIdx.x; t < n_tiles; t += BIN_THREADS) hist[t] = 0;
+  int *hist = lds;                                           // tpi
+  uint16_t *rank = reinterpret_cast<uint16_t *>(lds + d.tpi);  // per pair
+  for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) hist[t] = 0;
   __syncthreads();
-  const long long base = (long long)blockIdx.x * BIN_THREADS * BIN_PPT;
-  for (int k = 0; k < BIN_PPT; k++) {
-    const long long i = base + (long long)k * BIN_THREADS + threadIdx.x;
-    if (i >= n) break;
+  const BinBlock bb = bin_block(d);
+  for (int k = 0; k < d.ppt; k++) {
+    const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
+    if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
     if (r.cell < 0 || r.supp <= 0) continue;
     int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, pitch, lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > max_pairs_pt) continue;
+    point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) continue;
     int j = 0;
     for (int a = tu0; a <= tu1; a++)
       for (int b = tv0; b <= tv1; b++, j++)
-        rank[((k * BIN_THREADS) + threadIdx.x) * max_pairs_pt + j] =
-            (uint16_t)atomicAdd(&hist[a * ntx + b], 1);
+        rank[(k * BIN_THREADS + threadIdx.x) * d.mpp + j] =
+            (uint16_t)atomicAdd(&hist[a * d.ntx + b], 1);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < n_tiles; t += BIN_THREADS) {
+  const long long t0 = (long long)bb.inst * d.tpi;
+  for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) {
     const int c = hist[t];
-    hist[t] = c ? tile_start[t] + atomicAdd(&tile_cursor[t], c) : 0;
+    hist[t] = c ? d.tile_start[t0 + t] + atomicAdd(&d.tile_cursor[t0 + t], c) : 0;
   }
   __syncthreads();
-  for (int k = 0; k < BIN_PPT; k++) {
-    const long long i = base + (long long)k * BIN_THREADS + threadIdx.x;
-    if (i >= n) break;
+  for (int k = 0; k < d.ppt; k++) {
+    const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
+    if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
     if (r.cell < 0 || r.supp <= 0) continue;
     int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, pitch, lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > max_pairs_pt) continue;
+    point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) continue;
     int j = 0;
     for (int a = tu0; a <= tu1; a++)
       for (int b = tv0; b <= tv1; b++, j++) {
-        const long long pos =
-            (long long)hist[a * ntx + b] +
-            rank[((k * BIN_THREADS) + threadIdx.x) * max_pairs_pt + j];
-        if (pos < pair_cap) pairs[pos] = make_pair(u0, v0, r, (int)i, a, b, 1 << lt, j);
+        const long long pos = (long long)hist[a * d.ntx + b] +
+                              rank[(k * BIN_THREADS + threadIdx.x) * d.mpp + j];
+        if (pos < d.pair_cap) d.pairs[pos] = make_pair(u0, v0, r, i, a, b, 1 << d.lt, j);
         else atomicOr(err, ERR_CAPACITY);
       }
   }
 }
 
 // ---------------------------------------------------------------------------
-// work-item lookup: the largest t with item_start[t] <= item.  Tiles after
-// the owning tile start past `item`, so this is the owner (empty tiles
-// before it share smaller indices).
-__device__ __forceinline__ int find_tile(const int *__restrict__ item_start,
-                                         const int *__restrict__ tile_count,
-                                         int n_tiles, int item)
+// work items
+
+// The largest t with item_start[t] <= item.  Tiles after the owning tile
+// start past `item`, so this is the owner (empty tiles before it share
+// smaller indices).
+__device__ __forceinline__ int find_tile(const int *__restrict__ item_start, int n_tiles,
+                                         int item)
 {
   int lo = 0, hi = n_tiles - 1;
   while (lo < hi) {
@@ -258,33 +317,63 @@ __device__ __forceinline__ int find_tile(const int *__restrict__ item_start,
   return lo;
 }
 
-__device__ __forceinline__ int xcd_item(int b, int total)
+// Next work item of a workgroup (called by one thread).  The `total` items
+// are cut into 8 contiguous ranges, one per XCD: workgroups are dispatched
+// round-robin over the XCDs, so workgroup b runs on XCD b mod 8 and draws
+// from range b mod 8 first, then from the others in turn (`tried` counts
+// the ranges found empty).  Returns -1 when every range is used up; each
+// counter only grows, so every workgroup ends after at most 8 failed draws.
+__device__ __forceinline__ int draw_item(int *__restrict__ q, int total, int home, int &tried)
 {
-  // Workgroups are dispatched round-robin over the 8 XCDs (b mod 8); give
-  // each XCD a contiguous range of the `total` items so a tile's items and
-  // neighbouring tiles share an L2.  Blocks past the ranges return -1.
   const int per = (total + 7) >> 3;
-  const int k = b >> 3;
-  return k < per ? (b & 7) * per + k : -1;
+  while (tried < 8) {
+    const int x = (home + tried) & 7;
+    const int k = atomicAdd(&q[x], 1);
+    if (k < per && x * per + k < total) return x * per + k;
+    tried++;
+  }
+  return -1;
 }
 
 struct ItemRange {
-  int tile, begin, end, single;
+  int begin, end, single;  // pairs [begin, end); the tile has one item
+  int tu0, tv0;            // first grid row and column of the tile
+  long long goff;          // element offset of the instance's grid
 };
 
-__device__ __forceinline__ bool item_range(const TiledDev &d, ItemRange *ir)
+__device__ __forceinline__ ItemRange item_of(const TiledDev &d, int item)
 {
-  const int total = d.totals[1];
-  const int item = xcd_item(blockIdx.x, total);
-  if (item < 0 || item >= total) return false;
-  const int t = find_tile(d.item_start, d.tile_count, d.n_tiles, item);
+  const int t = find_tile(d.item_start, d.n_tiles, item);
   const int k = item - d.item_start[t];
   const int cnt = d.tile_count[t];
-  ir->tile = t;
-  ir->begin = d.tile_start[t] + k * d.chunk;
-  ir->end = d.tile_start[t] + min(cnt, (k + 1) * d.chunk);
-  ir->single = cnt <= d.chunk;
-  return true;
+  const int inst = t / d.tpi, lt = t - inst * d.tpi;
+  ItemRange ir;
+  ir.begin = d.tile_start[t] + k * d.chunk;
+  ir.end = d.tile_start[t] + min(cnt, (k + 1) * d.chunk);
+  ir.single = cnt <= d.chunk;
+  ir.tu0 = (lt / d.ntx) * d.T;
+  ir.tv0 = (lt % d.ntx) * d.T;
+  ir.goff = (long long)inst * d.gstride;
+  return ir;
+}
+
+// Work loop of a persistent workgroup: fn(item range) for every item it
+// draws.  The trailing barrier keeps the next draw (and the next item's
+// LDS writes) behind every wavefront's use of the current one.
+template <class F>
+__device__ __forceinline__ void for_each_item(const TiledDev &d, int *__restrict__ q, F fn)
+{
+  __shared__ int s_item;
+  const int total = d.totals[1];
+  int tried = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_item = draw_item(q, total, blockIdx.x & 7, tried);
+    __syncthreads();
+    const int item = s_item;
+    if (item < 0) break;
+    fn(item_of(d, item));
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -440,113 +529,115 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
   constexpr int CP = T * LS + 2;  // one copy (+ dummy cell)
   constexpr int DUMMY = T * LS;
   __shared__ float2 acc[4 * CP];
-  ItemRange ir;
-  if (!item_range(d, &ir)) return;
-  for (int q = threadIdx.x; q < 4 * CP; q += 256) acc[q] = make_float2(0.f, 0.f);
-  __syncthreads();
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   float2 *my_acc = acc + wave * CP;
-  const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
-  if (npw > 0) {
-    int4 rec, rec_n;
-    float2 mv, mv_n;
-    stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv);
-    stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n);
-    int k = 0, s0 = 0;
-    Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane);
-    float vx = rlf(mv.x, 0), vy = rlf(mv.y, 0);
-    float2 ta[GRP], tb[GRP];
-    load_group(ta, p, 0, rsrc);
-    bool more = true;
-    // One group: choose the next group (the same pair's next slabs, or the
-    // next pair's first), issue its taps into tn (past the end pn == p: a
-    // harmless reload), then update the current group's cells from tc while
-    // they fly.  Called alternately with the two buffers, so no register
-    // copy forces a wait on the loads in flight.
-    auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
-      Pair pn = p;
-      float vxn = vx, vyn = vy;
-      int kn = k, sn = s0 + GRP;
-      if (sn >= p.nslabs) {
-        kn = k + 1;
-        sn = 0;
-        if (kn < npw) {
-          const int j = kn & 63;
-          if (j == 0) {
-            rec = rec_n;
-            mv = mv_n;
-            stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n);
-          }
-          pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j),
+  for_each_item(d, d.queue, [&](const ItemRange &ir) {
+    for (int q = threadIdx.x; q < 4 * CP; q += 256) acc[q] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
+    if (npw > 0) {
+      int4 rec, rec_n;
+      float2 mv, mv_n;
+      stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv);
+      stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n);
+      int k = 0, s0 = 0;
+      Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0),
                                       lane);
-          vxn = rlf(mv.x, j);
-          vyn = rlf(mv.y, j);
+      float vx = rlf(mv.x, 0), vy = rlf(mv.y, 0);
+      float2 ta[GRP], tb[GRP];
+      load_group(ta, p, 0, rsrc);
+      bool more = true;
+      // One group: choose the next group (the same pair's next slabs, or the
+      // next pair's first), issue its taps into tn (past the end pn == p: a
+      // harmless reload), then update the current group's cells from tc
+      // while they fly.  Called alternately with the two buffers, so no
+      // register copy forces a wait on the loads in flight.
+      auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
+        Pair pn = p;
+        float vxn = vx, vyn = vy;
+        int kn = k, sn = s0 + GRP;
+        if (sn >= p.nslabs) {
+          kn = k + 1;
+          sn = 0;
+          if (kn < npw) {
+            const int j = kn & 63;
+            if (j == 0) {
+              rec = rec_n;
+              mv = mv_n;
+              stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n);
+            }
+            pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j),
+                                        rl(rec.w, j), lane);
+            vxn = rlf(mv.x, j);
+            vyn = rlf(mv.y, j);
+          }
         }
+        more = kn < npw;
+        load_group(tn, pn, sn, rsrc);
+        __builtin_amdgcn_sched_barrier(0);
+        const float a = vx, b = p.conj ? vy : -vy, c = p.conj ? -vx : vx, e = vy;
+        int li[GRP];
+        float2 x[GRP];
+#pragma unroll
+        for (int u = 0; u < GRP; u++) li[u] = group_lds(p, s0 + u, DUMMY);
+        // cells of one pair are distinct, so the reads may precede the writes
+#pragma unroll
+        for (int u = 0; u < GRP; u++) x[u] = my_acc[li[u]];
+#pragma unroll
+        for (int u = 0; u < GRP; u++) {
+          x[u].x = fmaf(a, tc[u].x, fmaf(b, tc[u].y, x[u].x));
+          x[u].y = fmaf(c, tc[u].y, fmaf(e, tc[u].x, x[u].y));
+        }
+#pragma unroll
+        for (int u = 0; u < GRP; u++) my_acc[li[u]] = x[u];
+        p = pn;
+        vx = vxn;
+        vy = vyn;
+        k = kn;
+        s0 = sn;
+      };
+      while (true) {
+        body(ta, tb);
+        if (!more) break;
+        body(tb, ta);
+        if (!more) break;
       }
-      more = kn < npw;
-      load_group(tn, pn, sn, rsrc);
-      __builtin_amdgcn_sched_barrier(0);
-      const float a = vx, b = p.conj ? vy : -vy, c = p.conj ? -vx : vx, e = vy;
-      int li[GRP];
-      float2 x[GRP];
+      // Never true (chunk >= 1): a use of both buffers on the loop exits
+      // keeps the compiler from sinking the prefetch loads past the exit
+      // tests.
+      if (d.chunk < 0) {
+        float2 z = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int u = 0; u < GRP; u++) li[u] = group_lds(p, s0 + u, DUMMY);
-      // cells of one pair are distinct, so the reads may precede the writes
-#pragma unroll
-      for (int u = 0; u < GRP; u++) x[u] = my_acc[li[u]];
-#pragma unroll
-      for (int u = 0; u < GRP; u++) {
-        x[u].x = fmaf(a, tc[u].x, fmaf(b, tc[u].y, x[u].x));
-        x[u].y = fmaf(c, tc[u].y, fmaf(e, tc[u].x, x[u].y));
+        for (int u = 0; u < GRP; u++) {
+          z.x += ta[u].x + tb[u].x;
+          z.y += ta[u].y + tb[u].y;
+        }
+        my_acc[lane] = z;
       }
-#pragma unroll
-      for (int u = 0; u < GRP; u++) my_acc[li[u]] = x[u];
-      p = pn;
-      vx = vxn;
-      vy = vyn;
-      k = kn;
-      s0 = sn;
-    };
-    while (true) {
-      body(ta, tb);
-      if (!more) break;
-      body(tb, ta);
-      if (!more) break;
     }
-    // Never true (chunk >= 1): a use of both buffers on the loop exits keeps
-    // the compiler from sinking the prefetch loads past the exit tests.
-    if (d.chunk < 0) {
-      float2 z = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int u = 0; u < GRP; u++) {
-        z.x += ta[u].x + tb[u].x;
-        z.y += ta[u].y + tb[u].y;
+    __syncthreads();
+    // flush: sum the four copies
+    const int gsz = d.grid_size;
+    float2 *gi = grid + ir.goff;
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+      const int r = q / T, cc = q - (q / T) * T;
+      const int gu = ir.tu0 + r, gv = ir.tv0 + cc;
+      if (gu >= gsz || gv >= gsz) continue;
+      const int l = r * LS + cc;
+      const float2 s0 = acc[l], s1 = acc[CP + l], s2 = acc[2 * CP + l], s3 = acc[3 * CP + l];
+      const float re = (s0.x + s1.x) + (s2.x + s3.x);
+      const float im = (s0.y + s1.y) + (s2.y + s3.y);
+      float2 *g = gi + (long long)gu * d.pitch + gv;
+      if (ir.single) {
+        *g = make_float2(re, im);
+      } else if (re != 0.f || im != 0.f) {
+        unsafeAtomicAdd(&g->x, re);
+        unsafeAtomicAdd(&g->y, im);
       }
-      my_acc[lane] = z;
     }
-  }
-  __syncthreads();
-  // flush: sum the four copies
-  const int tu0 = (ir.tile / d.ntx) * T, tv0 = (ir.tile % d.ntx) * T;
-  const int gsz = d.grid_size;
-  for (int q = threadIdx.x; q < T * T; q += 256) {
-    const int r = q / T, cc = q - (q / T) * T;
-    const int gu = tu0 + r, gv = tv0 + cc;
-    if (gu >= gsz || gv >= gsz) continue;
-    const int l = r * LS + cc;
-    const float2 s0 = acc[l], s1 = acc[CP + l], s2 = acc[2 * CP + l], s3 = acc[3 * CP + l];
-    const float re = (s0.x + s1.x) + (s2.x + s3.x);
-    const float im = (s0.y + s1.y) + (s2.y + s3.y);
-    float2 *g = grid + (long long)gu * d.pitch + gv;
-    if (ir.single) {
-      *g = make_float2(re, im);
-    } else if (re != 0.f || im != 0.f) {
-      unsafeAtomicAdd(&g->x, re);
-      unsafeAtomicAdd(&g->y, im);
-    }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -583,100 +674,102 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
   constexpr int LS = T + 1;
   constexpr int DUMMY = T * LS;
   __shared__ float2 tile[T * LS + 2];
-  ItemRange ir;
-  if (!item_range(d, &ir)) return;
-  const int tu0 = (ir.tile / d.ntx) * T, tv0 = (ir.tile % d.ntx) * T;
-  const int gsz = d.grid_size;
-  for (int q = threadIdx.x; q < T * T; q += 256) {
-    const int r = q / T, c = q - (q / T) * T;
-    const int gu = tu0 + r, gv = tv0 + c;
-    tile[r * LS + c] = (gu < gsz && gv < gsz) ? grid[(long long)gu * d.pitch + gv]
-                                              : make_float2(0.f, 0.f);
-  }
-  if (threadIdx.x == 0) tile[DUMMY] = make_float2(0.f, 0.f);
-  __syncthreads();
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
-  if (npw == 0) return;
-  int4 rec, rec_n;
-  float2 dm, dm_n;
-  stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm);
-  stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n);
-  int k = 0, s0 = 0;
-  Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane);
-  float2 ta[GRP], tb[GRP];
-  load_group(ta, p, 0, rsrc);
-  // P = sum gx*tx, Q = sum gy*ty, R = sum gx*ty, U = sum gy*tx
-  float P = 0.f, Q = 0.f, R = 0.f, U = 0.f;
-  bool more = true;
-  // one group; alternate buffers as in the gridder
-  auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
-    Pair pn = p;
-    int kn = k, sn = s0 + GRP;
-    const bool last = sn >= p.nslabs;
-    if (last) {
-      kn = k + 1;
-      sn = 0;
-      if (kn < npw) {
-        const int j = kn & 63;
-        if (j == 0) {
-          rec = rec_n;
-          stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n);
+  for_each_item(d, d.queue + 8, [&](const ItemRange &ir) {
+    const int gsz = d.grid_size;
+    const float2 *gi = grid + ir.goff;
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+      const int r = q / T, c = q - (q / T) * T;
+      const int gu = ir.tu0 + r, gv = ir.tv0 + c;
+      tile[r * LS + c] = (gu < gsz && gv < gsz) ? gi[(long long)gu * d.pitch + gv]
+                                                : make_float2(0.f, 0.f);
+    }
+    if (threadIdx.x == 0) tile[DUMMY] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
+    if (npw > 0) {
+      int4 rec, rec_n;
+      float2 dm, dm_n;
+      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm);
+      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n);
+      int k = 0, s0 = 0;
+      Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0),
+                                      lane);
+      float2 ta[GRP], tb[GRP];
+      load_group(ta, p, 0, rsrc);
+      // P = sum gx*tx, Q = sum gy*ty, R = sum gx*ty, U = sum gy*tx
+      float P = 0.f, Q = 0.f, R = 0.f, U = 0.f;
+      bool more = true;
+      // one group; alternate buffers as in the gridder
+      auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
+        Pair pn = p;
+        int kn = k, sn = s0 + GRP;
+        const bool last = sn >= p.nslabs;
+        if (last) {
+          kn = k + 1;
+          sn = 0;
+          if (kn < npw) {
+            const int j = kn & 63;
+            if (j == 0) {
+              rec = rec_n;
+              stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n);
+            }
+            pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j),
+                                        rl(rec.w, j), lane);
+          }
         }
-        pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j),
-                                    lane);
+        more = kn < npw;
+        load_group(tn, pn, sn, rsrc);
+        __builtin_amdgcn_sched_barrier(0);
+        float2 g[GRP];
+#pragma unroll
+        for (int u = 0; u < GRP; u++) g[u] = tile[group_lds(p, s0 + u, DUMMY)];
+#pragma unroll
+        for (int u = 0; u < GRP; u++) {
+          P = fmaf(g[u].x, tc[u].x, P);
+          Q = fmaf(g[u].y, tc[u].y, Q);
+          R = fmaf(g[u].x, tc[u].y, R);
+          U = fmaf(g[u].y, tc[u].x, U);
+        }
+        if (last) {
+          // tap t' = (tx, sg*ty): re = P - sg*Q, im = sg*R + U
+          float re = p.conj ? P + Q : P - Q;
+          float im = p.conj ? U - R : U + R;
+          re = wave_sum(re);
+          im = wave_sum(im);
+          // a point inside one tile is final; a split point leaves its part
+          // in its slot for k_degrid_reduce
+          if (lane == 0) {
+            float2 *o = p.whole ? vis + p.point
+                                : d.partial + (long long)p.point * d.mpp + p.slot;
+            *o = make_float2(re, im);
+          }
+          P = Q = R = U = 0.f;
+        }
+        p = pn;
+        k = kn;
+        s0 = sn;
+      };
+      while (true) {
+        body(ta, tb);
+        if (!more) break;
+        body(tb, ta);
+        if (!more) break;
+      }
+      // never true (chunk >= 1): keeps the prefetch loads from sinking
+      if (d.chunk < 0) {
+        float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < GRP; u++) {
+          z.x += ta[u].x + tb[u].x;
+          z.y += ta[u].y + tb[u].y;
+        }
+        vis[lane] = z;
       }
     }
-    more = kn < npw;
-    load_group(tn, pn, sn, rsrc);
-    __builtin_amdgcn_sched_barrier(0);
-    float2 g[GRP];
-#pragma unroll
-    for (int u = 0; u < GRP; u++) g[u] = tile[group_lds(p, s0 + u, DUMMY)];
-#pragma unroll
-    for (int u = 0; u < GRP; u++) {
-      P = fmaf(g[u].x, tc[u].x, P);
-      Q = fmaf(g[u].y, tc[u].y, Q);
-      R = fmaf(g[u].x, tc[u].y, R);
-      U = fmaf(g[u].y, tc[u].x, U);
-    }
-    if (last) {
-      // tap t' = (tx, sg*ty): re = P - sg*Q, im = sg*R + U
-      float re = p.conj ? P + Q : P - Q;
-      float im = p.conj ? U - R : U + R;
-      re = wave_sum(re);
-      im = wave_sum(im);
-      // a point inside one tile is final; a split point leaves its part in
-      // its slot for k_degrid_reduce
-      if (lane == 0) {
-        float2 *o = p.whole ? vis + p.point
-                            : d.partial + (long long)p.point * d.mpp + p.slot;
-        *o = make_float2(re, im);
-      }
-      P = Q = R = U = 0.f;
-    }
-    p = pn;
-    k = kn;
-    s0 = sn;
-  };
-  while (true) {
-    body(ta, tb);
-    if (!more) break;
-    body(tb, ta);
-    if (!more) break;
-  }
-  // never true (chunk >= 1): keeps the prefetch loads from sinking
-  if (d.chunk < 0) {
-    float2 z = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int u = 0; u < GRP; u++) {
-      z.x += ta[u].x + tb[u].x;
-      z.y += ta[u].y + tb[u].y;
-    }
-    vis[lane] = z;
-  }
+  });
 }
 
 // Split points: the sum of their per-tile parts in slot order (fixed order,
@@ -721,50 +814,98 @@ hipError_t dalloc(T **p, size_t n)
   return hipMalloc(reinterpret_cast<void **>(p), (n ? n : 1) * sizeof(T));
 }
 
-}  // namespace
-
-rcg_status tiled_blocks(const rcg_tiled_plan *pl, int *nb)
+// Persistent workgroups of a kernel: resident workgroups per CU x CUs,
+// a multiple of 8 (one share per XCD).
+hipError_t persistent_blocks(const void *kernel, int *nb)
 {
-  // upper bound on work items: one per tile plus one per `chunk` pairs
-  long long ub = (long long)pl->dev.n_tiles + pl->dev.pair_cap / pl->dev.chunk + 1;
-  ub = ((ub + 7) & ~7LL) + 8;
-  if (ub > INT_MAX) {
-    set_error(RCG_ERR_ARG, "too many work items (%lld)", ub);
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+  if (e != hipSuccess) return e;
+  *nb = std::max(8, (std::max(1, per_cu) * std::max(1, cus)) & ~7);
+  return hipSuccess;
+}
+
+template <int T>
+hipError_t tile_kernels(int *grid_blocks, int *degrid_blocks)
+{
+  hipError_t e = persistent_blocks(reinterpret_cast<const void *>(&k_grid_tiles<T>),
+                                   grid_blocks);
+  if (e == hipSuccess)
+    e = persistent_blocks(reinterpret_cast<const void *>(&k_degrid_tiles<T>), degrid_blocks);
+  return e;
+}
+
+rcg_status check_tap_table(const rcg_gcf *gcf, const char *who)
+{
+  if (gcf->n_taps <= 0 || gcf->n_taps >= (0x7fffffffLL >> 3)) {
+    set_error(RCG_ERR_ARG, "%s: tap table of %lld elements; the FP32 path needs "
+              "0 < n_taps < 2^28", who, (long long)gcf->n_taps);
     return RCG_ERR_ARG;
   }
-  *nb = (int)ub;
   return RCG_OK;
 }
+
+// Events around a plan's main kernel when timing is on.
+hipEvent_t *timing_slot(rcg_tiled_plan *pl, int dir)
+{
+  if (!pl->timing) return nullptr;
+  return pl->ev[dir][pl->n_ev[dir] % N_TIMED];
+}
+
+}  // namespace
 
 }  // namespace rcg
 
 using namespace rcg;
 
-extern "C" rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile,
-                                            int chunk, int max_support,
-                                            rcg_tiled_plan **out)
+extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_inst, int tile,
+                                                  int chunk, int max_support,
+                                                  rcg_tiled_plan **out)
 {
   clear_error();
-  if (!geom || !out) {
-    set_error(RCG_ERR_ARG, "rcg_tiled_plan_create: null argument");
+  if (!geoms || !out || n_inst < 1) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_plan_create: null argument or no instance");
     return RCG_ERR_ARG;
   }
   *out = nullptr;
-  rcg_status s = check_geom(geom);
-  if (s != RCG_OK) return s;
-  if (tile != 32 && tile != 64 && tile != 128) {
-    set_error(RCG_ERR_ARG, "tile must be 32, 64 or 128 (got %d)", tile);
+  long long n = 0;
+  std::vector<int> pt_off(n_inst + 1), blk_off(n_inst + 1);
+  for (int k = 0; k < n_inst; k++) {
+    rcg_status s = check_geom(&geoms[k]);
+    if (s != RCG_OK) return s;
+    if (geoms[k].grid_size != geoms[0].grid_size || geoms[k].grid_pitch != geoms[0].grid_pitch) {
+      set_error(RCG_ERR_ARG, "instance %d: every instance needs the grid shape of the first", k);
+      return RCG_ERR_ARG;
+    }
+    pt_off[k] = (int)n;
+    n += (long long)geoms[k].n_chunks * geoms[k].ts_ch;
+    if (n > INT_MAX) {
+      set_error(RCG_ERR_ARG, "more than 2^31-1 points in one plan");
+      return RCG_ERR_ARG;
+    }
+  }
+  pt_off[n_inst] = (int)n;
+  if (tile != 16 && tile != 32 && tile != 64) {
+    set_error(RCG_ERR_ARG, "tile must be 16, 32 or 64 (got %d)", tile);
     return RCG_ERR_ARG;
   }
-  if (chunk < 1 || max_support < 1 || max_support > MAX_SUPPORT) {
+  if (chunk < 1 || chunk > (1 << 20) || max_support < 1 || max_support > MAX_SUPPORT) {
     set_error(RCG_ERR_ARG, "chunk %d / max_support %d out of range", chunk, max_support);
     return RCG_ERR_ARG;
   }
-  const int ntx = (geom->grid_size + tile - 1) / tile;
-  const long long n_tiles = (long long)ntx * ntx;
-  if (n_tiles > MAX_BIN_TILES) {
-    set_error(RCG_ERR_ARG, "%lld tiles exceed the binning limit %d: use a larger tile",
-              n_tiles, MAX_BIN_TILES);
+  const int size = geoms[0].grid_size, pitch = geoms[0].grid_pitch;
+  const int ntx = (size + tile - 1) / tile;
+  const long long tpi = (long long)ntx * ntx;
+  if (tpi > MAX_BIN_TILES) {
+    set_error(RCG_ERR_ARG, "%lld tiles per grid exceed the binning limit %d: use a larger tile",
+              tpi, MAX_BIN_TILES);
+    return RCG_ERR_ARG;
+  }
+  if (tpi * n_inst > MAX_PLAN_TILES) {
+    set_error(RCG_ERR_ARG, "%lld tiles in the plan exceed %d: split the batch",
+              tpi * n_inst, MAX_PLAN_TILES);
     return RCG_ERR_ARG;
   }
   const int per_dim = (max_support - 1 + tile - 1) / tile + 1;
@@ -775,108 +916,152 @@ extern "C" rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile,
               max_support, max_pairs_pt, tile);
     return RCG_ERR_UNSUPPORTED;
   }
-  const long long n = (long long)geom->n_chunks * geom->ts_ch;
+  if (n * max_pairs_pt > INT_MAX) {
+    set_error(RCG_ERR_ARG, "%lld points x %d tiles per point exceed the pair index range: "
+              "split the batch", n, max_pairs_pt);
+    return RCG_ERR_ARG;
+  }
+  // points per binning thread: as many as the fill pass's LDS allows
+  int ppt = 4;
+  while (ppt > 0 && tpi * 4 + (long long)BIN_THREADS * ppt * max_pairs_pt * 2 > 160 * 1024)
+    ppt >>= 1;
+  if (ppt == 0) {
+    set_error(RCG_ERR_UNSUPPORTED, "binning LDS: %lld tiles and %d tiles per point do not fit",
+              tpi, max_pairs_pt);
+    return RCG_ERR_UNSUPPORTED;
+  }
+  int nblk = 0;
+  for (int k = 0; k < n_inst; k++) {
+    blk_off[k] = nblk;
+    nblk += ceil_div(pt_off[k + 1] - pt_off[k], (long long)BIN_THREADS * ppt);
+  }
+  blk_off[n_inst] = nblk;
+
   rcg_tiled_plan *pl = new (std::nothrow) rcg_tiled_plan();
   if (!pl) {
     set_error(RCG_ERR_NOMEM, "host allocation failed");
     return RCG_ERR_NOMEM;
   }
-  pl->geom = *geom;
+  pl->geoms.assign(geoms, geoms + n_inst);
   pl->n = n;
   pl->max_support = max_support;
   pl->max_pairs_pt = max_pairs_pt;
+  pl->bin_blocks = nblk;
   TiledDev &d = pl->dev;
   d.T = tile;
   d.lt = ilog2(tile);
   d.ntx = ntx;
-  d.n_tiles = (int)n_tiles;
+  d.tpi = (int)tpi;
+  d.n_inst = n_inst;
+  d.n_tiles = (int)(tpi * n_inst);
   d.chunk = chunk;
-  d.pitch = geom->grid_pitch;
-  d.grid_size = geom->grid_size;
+  d.pitch = pitch;
+  d.grid_size = size;
+  d.mpp = max_pairs_pt;
+  d.ppt = ppt;
+  d.gstride = (long long)size * pitch;
   d.pair_cap = n * max_pairs_pt;
+  const int n_seg = ceil_div(d.n_tiles, SCAN_SEG);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bin_fill),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     160 * 1024);
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e == hipSuccess)
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bin_count),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e == hipSuccess) {
+    switch (tile) {
+      case 16: e = tile_kernels<16>(&pl->grid_blocks, &pl->degrid_blocks); break;
+      case 32: e = tile_kernels<32>(&pl->grid_blocks, &pl->degrid_blocks); break;
+      default: e = tile_kernels<64>(&pl->grid_blocks, &pl->degrid_blocks); break;
+    }
+  }
+  if (e == hipSuccess) e = dalloc(&pl->geoms_dev, (size_t)n_inst);
+  if (e == hipSuccess) e = dalloc(&pl->offs, (size_t)2 * (n_inst + 1));
   if (e == hipSuccess) e = dalloc(&pl->pg, (size_t)n);
-  if (e == hipSuccess) e = dalloc(&d.tile_count, (size_t)n_tiles);
-  if (e == hipSuccess) e = dalloc(&d.tile_start, (size_t)n_tiles);
-  if (e == hipSuccess) e = dalloc(&d.tile_cursor, (size_t)n_tiles);
-  if (e == hipSuccess) e = dalloc(&d.item_start, (size_t)n_tiles);
+  if (e == hipSuccess) e = dalloc(&d.tile_count, (size_t)d.n_tiles);
+  if (e == hipSuccess) e = dalloc(&d.tile_start, (size_t)d.n_tiles);
+  if (e == hipSuccess) e = dalloc(&d.tile_cursor, (size_t)d.n_tiles);
+  if (e == hipSuccess) e = dalloc(&d.item_start, (size_t)d.n_tiles);
   if (e == hipSuccess) e = dalloc(&d.totals, 4);
+  if (e == hipSuccess) e = dalloc(&d.seg, (size_t)2 * n_seg);
+  if (e == hipSuccess) e = dalloc(&d.queue, (size_t)N_QUEUE);
   if (e == hipSuccess) e = dalloc(&pl->taps, 1);
   if (e == hipSuccess) e = dalloc(&pl->err, 1);
   if (e == hipSuccess) e = dalloc(&d.pairs, (size_t)d.pair_cap);
-  d.mpp = max_pairs_pt;
-  if (e == hipSuccess) e = dalloc(&d.partial, (size_t)n * max_pairs_pt);
+  if (e == hipSuccess) e = dalloc(&d.partial, (size_t)d.pair_cap);
+  if (e == hipSuccess)
+    e = hipMemcpy(pl->geoms_dev, geoms, sizeof(rcg_geom) * n_inst, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(pl->offs, pt_off.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(pl->offs + n_inst + 1, blk_off.data(), sizeof(int) * (n_inst + 1),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     rcg_tiled_plan_destroy(pl);
-    set_error(RCG_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
+    set_error(RCG_ERR_NOMEM, "plan set-up failed: %s", hipGetErrorString(e));
     return RCG_ERR_NOMEM;
   }
+  d.pt_off = pl->offs;
+  d.blk_off = pl->offs + n_inst + 1;
   *out = pl;
   return RCG_OK;
+}
+
+extern "C" rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile, int chunk,
+                                            int max_support, rcg_tiled_plan **out)
+{
+  return rcg_tiled_plan_create_batch(geom, 1, tile, chunk, max_support, out);
 }
 
 extern "C" rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *pl)
 {
   if (!pl) return RCG_OK;
   TiledDev &d = pl->dev;
-  void *ptrs[] = {pl->pg, d.tile_count, d.tile_start, d.tile_cursor,
-                  d.item_start, d.totals, pl->taps, pl->err, d.pairs, d.partial};
+  void *ptrs[] = {pl->geoms_dev, pl->offs, pl->pg, d.tile_count, d.tile_start,
+                  d.tile_cursor, d.item_start, d.totals, d.seg, d.queue, pl->taps, pl->err,
+                  d.pairs, d.partial};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (int dir = 0; dir < 2; dir++)
+    for (int k = 0; k < N_TIMED; k++)
+      for (int j = 0; j < 2; j++)
+        if (pl->ev[dir][k][j]) (void)hipEventDestroy(pl->ev[dir][k][j]);
   delete pl;
   return RCG_OK;
 }
 
-extern "C" rcg_status rcg_tiled_bin(rcg_tiled_plan *pl, const rcg_geom *geom,
-                                    const rcg_gcf *gcf, void *stream)
+extern "C" rcg_status rcg_tiled_bin(rcg_tiled_plan *pl, const rcg_gcf *gcf, void *stream)
 {
   clear_error();
-  if (!pl || !geom || !gcf) {
+  if (!pl || !gcf) {
     set_error(RCG_ERR_ARG, "rcg_tiled_bin: null argument");
     return RCG_ERR_ARG;
   }
-  if ((long long)geom->n_chunks * geom->ts_ch != pl->n ||
-      geom->grid_size != pl->geom.grid_size || geom->grid_pitch != pl->geom.grid_pitch) {
-    set_error(RCG_ERR_ARG, "geometry does not match the plan");
-    return RCG_ERR_ARG;
-  }
+  rcg_status s = check_gcf(gcf);
+  if (s != RCG_OK) return s;
   hipStream_t st = (hipStream_t)stream;
   TiledDev &d = pl->dev;
-  const long long n = pl->n;
-  hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, st, d.tile_count, 0,
+  hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, d.tile_count, 0,
                      (long long)d.n_tiles);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.totals, 0, 4LL);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st,
                      reinterpret_cast<int *>(pl->taps), 0, 2LL);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, pl->err, 0, 1LL);
   RCG_HIP_TRY(hipGetLastError());
-  rcg_status s = rcg_pregrid(geom, gcf, pl->pg, pl->taps, pl->err, stream);
+  if (pl->n == 0) return RCG_OK;
+  s = launch_pregrid_batch(pl->geoms_dev, d.pt_off, d.n_inst, pl->n, gcf, pl->pg, pl->taps,
+                           pl->err, st);
   if (s != RCG_OK) return s;
-  if (n == 0) return RCG_OK;
-  const int nb = ceil_div(n, (long long)BIN_THREADS * BIN_PPT);
-  const size_t lds_count = (size_t)d.n_tiles * sizeof(int);
-  hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(BIN_THREADS), lds_count, st,
-                     pl->pg, n, d.pitch, d.lt, d.ntx, d.n_tiles, pl->max_pairs_pt,
-                     d.tile_count, pl->err);
-  RCG_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(SCAN_THREADS), 0, st, d.tile_count,
-                     d.n_tiles, d.chunk, d.tile_start, d.item_start, d.tile_cursor,
-                     d.totals);
-  RCG_HIP_TRY(hipGetLastError());
-  const size_t lds_fill = lds_count + (size_t)BIN_THREADS * BIN_PPT *
-                                          pl->max_pairs_pt * sizeof(uint16_t);
-  if (lds_fill > 160 * 1024) {
-    set_error(RCG_ERR_ARG, "binning needs %zu bytes of LDS", lds_fill);
-    return RCG_ERR_ARG;
-  }
-  hipLaunchKernelGGL(k_bin_fill, dim3(nb), dim3(BIN_THREADS), lds_fill, st, pl->pg,
-                     n, d.pitch, d.lt, d.ntx, d.n_tiles, pl->max_pairs_pt,
-                     d.tile_start, d.tile_cursor, d.pairs, d.pair_cap, pl->err);
+  const size_t lds_count = (size_t)d.tpi * sizeof(int);
+  const size_t lds_fill =
+      lds_count + (size_t)BIN_THREADS * d.ppt * d.mpp * sizeof(uint16_t);
+  const int n_seg = ceil_div(d.n_tiles, SCAN_SEG);
+  hipLaunchKernelGGL(k_bin_count, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_count, st,
+                     pl->pg, d, pl->err);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(n_seg), dim3(SCAN_SEG), 0, st, d);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_SEG), 0, st, d, n_seg);
+  hipLaunchKernelGGL(k_scan_apply, dim3(n_seg), dim3(SCAN_SEG), 0, st, d);
+  hipLaunchKernelGGL(k_bin_fill, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_fill, st,
+                     pl->pg, d, pl->err);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
@@ -890,33 +1075,32 @@ extern "C" rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
     set_error(RCG_ERR_ARG, "rcg_grid_tiled_f32: null argument");
     return RCG_ERR_ARG;
   }
-  if (gcf->n_taps <= 0 || gcf->n_taps >= (0x7fffffffLL >> 3)) {
-    set_error(RCG_ERR_ARG, "tap table of %lld elements: the FP32 path needs 0 < n_taps < 2^28",
-              (long long)gcf->n_taps);
-    return RCG_ERR_ARG;
-  }
+  rcg_status s = check_tap_table(gcf, "rcg_grid_tiled_f32");
+  if (s != RCG_OK) return s;
   hipStream_t st = (hipStream_t)stream;
   const TiledDev &d = pl->dev;
-  const long long cells = (long long)d.grid_size * d.pitch * 2;
   hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st,
-                     reinterpret_cast<float *>(grid), cells);
+                     reinterpret_cast<float *>(grid), d.gstride * d.n_inst * 2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue, 0, 8LL);
   RCG_HIP_TRY(hipGetLastError());
   if (pl->n == 0) return RCG_OK;
-  int nb;
-  rcg_status s = tiled_blocks(pl, &nb);
-  if (s != RCG_OK) return s;
   const float2 *v = reinterpret_cast<const float2 *>(vis);
   const float2 *t = reinterpret_cast<const float2 *>(gcf->data);
   const long long nt = gcf->n_taps;
   float2 *g = reinterpret_cast<float2 *>(grid);
+  const dim3 nb(pl->grid_blocks);
+  hipEvent_t *ev = timing_slot(pl, 0);
+  if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
   switch (d.T) {
-    case 32: hipLaunchKernelGGL(k_grid_tiles<32>, dim3(nb), dim3(256), 0, st, d, v, t, nt, g); break;
-    case 64: hipLaunchKernelGGL(k_grid_tiles<64>, dim3(nb), dim3(256), 0, st, d, v, t, nt, g); break;
-    default:
-      set_error(RCG_ERR_UNSUPPORTED, "tile %d not supported by the FP32 gridder", d.T);
-      return RCG_ERR_UNSUPPORTED;
+    case 16: hipLaunchKernelGGL(k_grid_tiles<16>, nb, dim3(256), 0, st, d, v, t, nt, g); break;
+    case 32: hipLaunchKernelGGL(k_grid_tiles<32>, nb, dim3(256), 0, st, d, v, t, nt, g); break;
+    default: hipLaunchKernelGGL(k_grid_tiles<64>, nb, dim3(256), 0, st, d, v, t, nt, g); break;
   }
   RCG_HIP_TRY(hipGetLastError());
+  if (ev) {
+    RCG_HIP_TRY(hipEventRecord(ev[1], st));
+    pl->n_ev[0]++;
+  }
   return RCG_OK;
 }
 
@@ -929,32 +1113,32 @@ extern "C" rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gc
     set_error(RCG_ERR_ARG, "rcg_degrid_tiled_f32: null argument");
     return RCG_ERR_ARG;
   }
-  if (gcf->n_taps <= 0 || gcf->n_taps >= (0x7fffffffLL >> 3)) {
-    set_error(RCG_ERR_ARG, "tap table of %lld elements: the FP32 path needs 0 < n_taps < 2^28",
-              (long long)gcf->n_taps);
-    return RCG_ERR_ARG;
-  }
+  rcg_status s = check_tap_table(gcf, "rcg_degrid_tiled_f32");
+  if (s != RCG_OK) return s;
   hipStream_t st = (hipStream_t)stream;
   const TiledDev &d = pl->dev;
   hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, st,
                      reinterpret_cast<float *>(vis), pl->n * 2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue + 8, 0, 8LL);
   RCG_HIP_TRY(hipGetLastError());
   if (pl->n == 0) return RCG_OK;
-  int nb;
-  rcg_status s = tiled_blocks(pl, &nb);
-  if (s != RCG_OK) return s;
   const float2 *g = reinterpret_cast<const float2 *>(grid);
   const float2 *t = reinterpret_cast<const float2 *>(gcf->data);
   const long long nt = gcf->n_taps;
   float2 *v = reinterpret_cast<float2 *>(vis);
+  const dim3 nb(pl->degrid_blocks);
+  hipEvent_t *ev = timing_slot(pl, 1);
+  if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
   switch (d.T) {
-    case 32: hipLaunchKernelGGL(k_degrid_tiles<32>, dim3(nb), dim3(256), 0, st, d, g, t, nt, v); break;
-    case 64: hipLaunchKernelGGL(k_degrid_tiles<64>, dim3(nb), dim3(256), 0, st, d, g, t, nt, v); break;
-    default:
-      set_error(RCG_ERR_UNSUPPORTED, "tile %d not supported by the FP32 degridder", d.T);
-      return RCG_ERR_UNSUPPORTED;
+    case 16: hipLaunchKernelGGL(k_degrid_tiles<16>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+    case 32: hipLaunchKernelGGL(k_degrid_tiles<32>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+    default: hipLaunchKernelGGL(k_degrid_tiles<64>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
   }
   RCG_HIP_TRY(hipGetLastError());
+  if (ev) {
+    RCG_HIP_TRY(hipEventRecord(ev[1], st));
+    pl->n_ev[1]++;
+  }
   hipLaunchKernelGGL(k_degrid_reduce, dim3((unsigned)((pl->n + 255) / 256)), dim3(256), 0, st,
                      d, pl->pg, pl->n, v);
   RCG_HIP_TRY(hipGetLastError());
@@ -982,5 +1166,125 @@ extern "C" rcg_status rcg_tiled_stats(rcg_tiled_plan *pl, unsigned long long *ta
   if (n_pairs) *n_pairs = tot[0];
   if (n_items) *n_items = tot[1];
   if (err) *err = e;
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_tiled_timing(rcg_tiled_plan *pl, int enable)
+{
+  clear_error();
+  if (!pl) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_timing: null plan");
+    return RCG_ERR_ARG;
+  }
+  if (enable && !pl->ev[0][0][0]) {
+    for (int dir = 0; dir < 2; dir++)
+      for (int k = 0; k < N_TIMED; k++)
+        for (int j = 0; j < 2; j++) RCG_HIP_TRY(hipEventCreate(&pl->ev[dir][k][j]));
+  }
+  pl->timing = enable != 0;
+  pl->n_ev[0] = pl->n_ev[1] = 0;
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_tiled_kernel_times(rcg_tiled_plan *pl, int which, float *ms,
+                                             int max, int *count)
+{
+  clear_error();
+  if (!pl || !ms || !count || which < 0 || which > 1 || max < 0) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_kernel_times: bad argument");
+    return RCG_ERR_ARG;
+  }
+  const int rec = pl->n_ev[which];
+  const int n = std::min(std::min(rec, N_TIMED), max);
+  for (int k = 0; k < n; k++) {
+    hipEvent_t *e = pl->ev[which][(rec - n + k) % N_TIMED];
+    RCG_HIP_TRY(hipEventSynchronize(e[1]));
+    RCG_HIP_TRY(hipEventElapsedTime(&ms[k], e[0], e[1]));
+  }
+  *count = n;
+  pl->n_ev[which] = 0;
+  return RCG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graphs
+
+struct rcg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" rcg_status rcg_graph_destroy(rcg_graph *g)
+{
+  if (!g) return RCG_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_tiled_graph_create(rcg_tiled_plan *pl, const rcg_gcf *gcf,
+                                             const rcg_c64 *vis, rcg_c64 *grid,
+                                             rcg_c64 *vis_out, int stages, rcg_graph **out)
+{
+  clear_error();
+  if (!pl || !gcf || !out || (stages & ~7) || !stages ||
+      ((stages & RCG_STAGE_GRID) && (!vis || !grid)) ||
+      ((stages & RCG_STAGE_DEGRID) && (!grid || !vis_out))) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_graph_create: bad argument");
+    return RCG_ERR_ARG;
+  }
+  *out = nullptr;
+  rcg_graph *g = new (std::nothrow) rcg_graph();
+  if (!g) {
+    set_error(RCG_ERR_NOMEM, "host allocation failed");
+    return RCG_ERR_NOMEM;
+  }
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    if (st) (void)hipStreamDestroy(st);
+    delete g;
+    set_error(RCG_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+    return RCG_ERR_HIP;
+  }
+  const bool timing = pl->timing;
+  pl->timing = false;  // no event records inside the graph
+  rcg_status s = RCG_OK;
+  if (stages & RCG_STAGE_BIN) s = rcg_tiled_bin(pl, gcf, st);
+  if (s == RCG_OK && (stages & RCG_STAGE_GRID)) s = rcg_grid_tiled_f32(pl, gcf, vis, grid, st);
+  if (s == RCG_OK && (stages & RCG_STAGE_DEGRID))
+    s = rcg_degrid_tiled_f32(pl, gcf, grid, vis_out, st);
+  pl->timing = timing;
+  e = hipStreamEndCapture(st, &g->graph);
+  if (s == RCG_OK && e != hipSuccess) {
+    set_error(RCG_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+    s = RCG_ERR_HIP;
+  }
+  if (s == RCG_OK) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      set_error(RCG_ERR_HIP, "graph instantiation: %s", hipGetErrorString(e));
+      s = RCG_ERR_HIP;
+    }
+  }
+  (void)hipStreamDestroy(st);
+  if (s != RCG_OK) {
+    rcg_graph_destroy(g);
+    return s;
+  }
+  *out = g;
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_graph_launch(rcg_graph *g, void *stream)
+{
+  clear_error();
+  if (!g || !g->exec) {
+    set_error(RCG_ERR_ARG, "rcg_graph_launch: null graph");
+    return RCG_ERR_ARG;
+  }
+  RCG_HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
   return RCG_OK;
 }

@@ -136,23 +136,40 @@ def degrid_plain_f64(geom: DeviceGeom, gcf: DeviceGCF, pg: torch.Tensor,
     return vis
 
 
+STAGE_BIN, STAGE_GRID, STAGE_DEGRID = 1, 2, 4
+
+
 class TiledPlan:
     """A tiled FP32 gridding plan (rcg_tiled_plan): pregrid + uv-tile bins for
-    one geometry, reused by grid and degrid.  Device scratch is owned by the
-    library and released by close()."""
+    one geometry or a batch of instances (a list of geometries with one grid
+    shape), reused by grid and degrid.  For a batch, visibilities are the
+    instances' points concatenated in order and the grid tensor is
+    [n_inst, grid_size, grid_pitch].  Device scratch is owned by the library
+    and released by close()."""
 
-    def __init__(self, geom: DeviceGeom, tile: int = 64, chunk: int = 512,
+    def __init__(self, geom, tile: int = 32, chunk: int = 512,
                  max_support: int | None = None):
-        self.geom = geom
+        geoms = list(geom) if isinstance(geom, (list, tuple)) else [geom]
+        self.batch = isinstance(geom, (list, tuple))
+        self.geoms = geoms
+        self.geom = geoms[0]
+        self.n = sum(g.n for g in geoms)
+        self.n_inst = len(geoms)
         if max_support is None:
-            max_support = int(geom.chunk_supps.max().item()) if geom.n_chunks else 1
+            max_support = max((int(g.chunk_supps.max().item()) for g in geoms if g.n_chunks),
+                              default=1)
         self.tile, self.chunk, self.max_support = tile, chunk, max(1, max_support)
         h = C.c_void_p()
-        g = geom.struct()
-        _lib.check(_lib.lib().rcg_tiled_plan_create(C.byref(g), tile, chunk,
-                                                    self.max_support, C.byref(h)),
+        arr = (_lib.Geom * len(geoms))(*[g.struct() for g in geoms])
+        _lib.check(_lib.lib().rcg_tiled_plan_create_batch(arr, len(geoms), tile, chunk,
+                                                          self.max_support, C.byref(h)),
                    "rcg_tiled_plan_create")
         self._h = h
+
+    @property
+    def grid_shape(self) -> tuple:
+        g = (self.geom.grid_size, self.geom.grid_pitch)
+        return (self.n_inst,) + g if self.batch else g
 
     def close(self) -> None:
         if self._h:
@@ -166,19 +183,30 @@ class TiledPlan:
             pass
 
     def bin(self, gcf: DeviceGCF, stream=None) -> None:
-        g, t = self.geom.struct(), gcf.struct()
-        _lib.check(_lib.lib().rcg_tiled_bin(self._h, C.byref(g), C.byref(t),
-                                            _stream(stream)), "rcg_tiled_bin")
+        t = gcf.struct()
+        _lib.check(_lib.lib().rcg_tiled_bin(self._h, C.byref(t), _stream(stream)),
+                   "rcg_tiled_bin")
+
+    def _check_vis(self, vis: torch.Tensor) -> None:
+        if vis.dtype != torch.complex64:
+            raise TypeError("FP32 path takes complex64 visibilities")
+        if vis.numel() != self.n or not vis.is_contiguous():
+            raise ValueError("vis must be a contiguous tensor of n points")
+
+    def _check_grid(self, grid: torch.Tensor) -> None:
+        if grid.dtype != torch.complex64:
+            raise TypeError("FP32 path takes a complex64 grid")
+        if tuple(grid.shape) != self.grid_shape or not grid.is_contiguous():
+            raise ValueError(f"grid must be a contiguous {self.grid_shape} tensor")
 
     def grid(self, gcf: DeviceGCF, vis: torch.Tensor, grid: torch.Tensor | None = None,
              stream=None) -> torch.Tensor:
-        if vis.dtype != torch.complex64 or gcf.data.dtype != torch.complex64:
-            raise TypeError("FP32 path takes complex64 visibilities and taps")
-        if vis.numel() != self.geom.n or not vis.is_contiguous():
-            raise ValueError("vis must be a contiguous tensor of n points")
+        if gcf.data.dtype != torch.complex64:
+            raise TypeError("FP32 path takes complex64 taps")
+        self._check_vis(vis)
         if grid is None:
-            grid = torch.empty((self.geom.grid_size, self.geom.grid_pitch),
-                               dtype=torch.complex64, device=vis.device)
+            grid = torch.empty(self.grid_shape, dtype=torch.complex64, device=vis.device)
+        self._check_grid(grid)
         t = gcf.struct()
         _lib.check(_lib.lib().rcg_grid_tiled_f32(self._h, C.byref(t), vis.data_ptr(),
                                                  grid.data_ptr(), _stream(stream)),
@@ -187,13 +215,12 @@ class TiledPlan:
 
     def degrid(self, gcf: DeviceGCF, grid: torch.Tensor, vis: torch.Tensor | None = None,
                stream=None) -> torch.Tensor:
-        if grid.dtype != torch.complex64 or gcf.data.dtype != torch.complex64:
-            raise TypeError("FP32 path takes a complex64 grid and taps")
-        if tuple(grid.shape) != (self.geom.grid_size, self.geom.grid_pitch) or \
-                not grid.is_contiguous():
-            raise ValueError("grid shape does not match the plan")
+        if gcf.data.dtype != torch.complex64:
+            raise TypeError("FP32 path takes complex64 taps")
+        self._check_grid(grid)
         if vis is None:
-            vis = torch.empty(self.geom.n, dtype=torch.complex64, device=grid.device)
+            vis = torch.empty(self.n, dtype=torch.complex64, device=grid.device)
+        self._check_vis(vis)
         t = gcf.struct()
         _lib.check(_lib.lib().rcg_degrid_tiled_f32(self._h, C.byref(t), grid.data_ptr(),
                                                    vis.data_ptr(), _stream(stream)),
@@ -210,3 +237,57 @@ class TiledPlan:
                                               _stream(stream)), "rcg_tiled_stats")
         return {"taps": taps.value, "pairs": pairs.value, "items": items.value,
                 "err": err.value}
+
+    def timing(self, enable: bool) -> None:
+        _lib.check(_lib.lib().rcg_tiled_timing(self._h, int(enable)), "rcg_tiled_timing")
+
+    def kernel_times(self, which: str) -> list[float]:
+        """Durations (ms) of the main grid ('grid') or degrid ('degrid')
+        kernel of the calls made since timing was switched on (at most 64)."""
+        buf = (C.c_float * 64)()
+        n = C.c_int()
+        _lib.check(_lib.lib().rcg_tiled_kernel_times(self._h, 0 if which == "grid" else 1,
+                                                     buf, 64, C.byref(n)),
+                   "rcg_tiled_kernel_times")
+        return [buf[k] for k in range(n.value)]
+
+    def capture(self, gcf: DeviceGCF, stages: int, vis: torch.Tensor | None = None,
+                grid: torch.Tensor | None = None,
+                vis_out: torch.Tensor | None = None) -> "Graph":
+        """A HIP graph of the chosen stages (STAGE_* bits) on these buffers."""
+        if vis is not None:
+            self._check_vis(vis)
+        if vis_out is not None:
+            self._check_vis(vis_out)
+        if grid is not None:
+            self._check_grid(grid)
+        t = gcf.struct()
+        h = C.c_void_p()
+        ptr = lambda x: x.data_ptr() if x is not None else None  # noqa: E731
+        _lib.check(_lib.lib().rcg_tiled_graph_create(self._h, C.byref(t), ptr(vis), ptr(grid),
+                                                     ptr(vis_out), stages, C.byref(h)),
+                   "rcg_tiled_graph_create")
+        return Graph(h, keep=(self, gcf, vis, grid, vis_out))
+
+
+class Graph:
+    """An executable HIP graph (rcg_graph); keeps its plan and buffers alive."""
+
+    def __init__(self, h: C.c_void_p, keep=()):
+        self._h = h
+        self._keep = keep
+
+    def launch(self, stream=None) -> None:
+        _lib.check(_lib.lib().rcg_graph_launch(self._h, _stream(stream)), "rcg_graph_launch")
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().rcg_graph_destroy(self._h)
+            self._h = C.c_void_p()
+        self._keep = ()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

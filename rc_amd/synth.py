@@ -140,6 +140,53 @@ def wproj(seed: int = 2, n_ant: int = 256, n_times: int = 306,
     return vc, gcf, grid_size
 
 
+def snapshots(seed: int = 4, n_inst: int = 64, grid_size: int = 2048, n_planes: int = 16,
+              s_min: int = 8, s_max: int = 48, median: float = 400_000.0,
+              sigma: float = 1.0, clip: tuple[int, int] = (20_000, 5_000_000),
+              n_ant: int = 256, minutes: float = 10.0, hours: float = 4.0,
+              vis_dtype=np.complex64, gcf_dtype=np.complex128
+              ) -> tuple[list[VisChunks], GCFTable, int]:
+    """Uneven batch of snapshot instances of the 256-antenna array: instance
+    k holds n_k points (lognormal, median `median`, log-sigma `sigma`,
+    clipped to `clip`) of random baselines at random times within a window
+    of `minutes` centred on an hour angle uniform over +-hours/2 (dec -45 deg,
+    1.4 GHz).  All instances share the grid size, scale, w step and the GCF
+    table (n_planes |w|-planes, supports s_min..s_max, half table)."""
+    rng = np.random.default_rng(seed)
+    ant = antenna_layout(rng, n_ant)
+    lam = C_LIGHT / 1.4e9
+    lat, dec = math.radians(-30.7), math.radians(-45.0)
+    e, nn = ant[:, 0], ant[:, 1]
+    X, Y, Z = -math.sin(lat) * nn, e, math.cos(lat) * nn
+    i, j = np.triu_indices(n_ant, k=1)
+    bx, by, bz = X[j] - X[i], Y[j] - Y[i], Z[j] - Z[i]
+    bmax = float(np.max(np.sqrt(bx ** 2 + by ** 2 + bz ** 2))) / lam
+    scale = (grid_size / 2 - 1 - s_max / 2) / bmax
+    wstep = bmax / (n_planes - 1)
+    sup = np.asarray(wplane_supports(n_planes, s_min, s_max), np.int32)
+    counts = np.clip(np.rint(median * np.exp(sigma * rng.standard_normal(n_inst))),
+                     clip[0], clip[1]).astype(np.int64)
+    out = []
+    for k in range(n_inst):
+        n = int(counts[k])
+        bl = rng.integers(0, bx.size, n)
+        h0 = rng.uniform(-hours / 2, hours / 2)
+        H = (h0 + rng.uniform(-0.5, 0.5, n) * minutes / 60.0) * (2 * math.pi / 24.0)
+        sH, cH = np.sin(H), np.cos(H)
+        x, y, z = bx[bl], by[bl], bz[bl]
+        uvw = np.empty((n, 3))
+        uvw[:, 0] = (sH * x + cH * y) / lam
+        uvw[:, 1] = (-math.sin(dec) * cH * x + math.sin(dec) * sH * y + math.cos(dec) * z) / lam
+        uvw[:, 2] = (math.cos(dec) * cH * x - math.cos(dec) * sH * y + math.sin(dec) * z) / lam
+        plane = np.minimum(np.rint(np.abs(uvw[:, 2]) / wstep).astype(np.int64), n_planes - 1)
+        vis = (rng.standard_normal(n, dtype=np.float32)
+               + 1j * rng.standard_normal(n, dtype=np.float32)).astype(vis_dtype)
+        out.append(VisChunks(uvw=uvw.reshape(n, 1, 3), vis=vis.reshape(n, 1),
+                             supps=sup[plane], scale=scale, wstep=wstep))
+    gcf = wplane_gcf(n_planes, s_min, s_max, half=True, dtype=gcf_dtype)
+    return out, gcf, grid_size
+
+
 def random_small(seed: int, grid_size: int = 64, chunks: int = 12,
                  ts_ch: int = 9, max_plane: int = 2, half: bool = False,
                  supports: tuple[int, ...] | None = None,

@@ -41,7 +41,7 @@ def run_tiled(vc, gcf, N, tile, chunk, grid_in=None):
     return out
 
 
-@pytest.mark.parametrize("tile", [32, 64])
+@pytest.mark.parametrize("tile", [16, 32, 64])
 @pytest.mark.parametrize("chunk", [3, 512])
 @pytest.mark.parametrize("seed,half,cs", [(0, False, "min"), (1, True, "min"),
                                           (2, True, "random")])
