@@ -150,3 +150,16 @@ def test_plan_rejects_mixed_grid_shapes(hip_device):
     from rc_amd import _lib
     with pytest.raises(_lib.RcgError, match="grid shape"):
         D.TiledPlan([g0, g1], tile=32, chunk=64, max_support=24)
+
+
+def test_autotune_picks_a_timed_candidate(hip_device):
+    from rc_amd import autotune
+    vcs, gcf, N = small_batch(seed=12, n_inst=3, empty=False)
+    geoms, dg, vis = upload(vcs, gcf, N)
+    res = autotune.autotune(geoms, dg, vis, tiles=(16, 32, 64), chunks=(64, 256), reps=1)
+    timed = [r for r in res["table"] if r["ms_step"] is not None]
+    assert len(res["table"]) == 6 and timed
+    assert (res["tile"], res["chunk"]) in [(r["tile"], r["chunk"]) for r in timed]
+    assert res["ms_step"] == min(r["ms_step"] for r in timed)
+    # memoised per workload signature
+    assert autotune.autotune(geoms, dg, vis) is res
