@@ -55,6 +55,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel: str) -> int | None:
+    """HBM bytes per dispatch of `kernel` at the default workload, from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed under profiles/
+    (lines 'traffic_bytes <kernel> <bytes>'), or None."""
+    p = ROOT / "profiles" / "wproj_tiled_traffic.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
 def cpu_baseline(vc, gcf, N, budget_s: float) -> dict:
     """The reference's own CPU gridder + degridder (compiled from its sources
     with its release flags, oracle/_ref/libms4ref_fast.so), else the oracle
@@ -318,10 +332,18 @@ def main() -> int:
     roof = {"bound": "hbm", "kernel": f"{dom}<{args.tile}>", "kernel_ms": round(t_dom, 4),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None, "traffic_source": None,
             "valu_tflops": round(8 * taps / (t_dom / 1e3) / 1e12, 3),
             "valu_frac": round(8 * taps / (t_dom / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)}
 
+    # measured HBM bytes of this kernel (profiled at tile 32, chunk 512 on the
+    # default workload only)
+    if args.tile == 32 and args.chunk == 512 and args.n_times == 306:
+        tb = profiled_traffic(f"{dom}<{args.tile}>")
+        if tb is not None:
+            roof["traffic"] = tb
+            roof["traffic_source"] = ("profiles/wproj_tiled_traffic.md: rocprofv3 "
+                                      "(2*FETCH_SIZE + WRITE_SIZE) KiB per dispatch")
     cpu = None
     if not args.no_cpu:
         try:
