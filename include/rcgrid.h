@@ -232,6 +232,13 @@ rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
 rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                                 const rcg_c64 *grid, rcg_c64 *vis,
                                 void *stream);
+/* The Hermitian adjoint of rcg_degrid_tiled_f32: gridding with conjugated
+ * taps (the reference's degridder is the plain transpose of its gridder,
+ * SG:368-431, so the adjoint conjugates).  Used as the backward pass of a
+ * measurement operator. */
+rcg_status rcg_grid_tiled_adjoint_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                                      const rcg_c64 *vis, rcg_c64 *grid,
+                                      void *stream);
 /* Synchronises `stream`; returns the in-bound tap count, pair and work-item
  * counts and the device error word of the last rcg_tiled_bin (bit 0: a point
  * referenced a layer outside the table; bit 1: pair buffer overflow). */

@@ -80,6 +80,7 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_graph_launch": (C.c_int, [vp, vp]),
     "rcg_graph_destroy": (C.c_int, [vp]),
     "rcg_grid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
+    "rcg_grid_tiled_adjoint_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_degrid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_tiled_stats": (C.c_int, [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_longlong),
                                   C.POINTER(C.c_longlong), C.POINTER(C.c_int), vp]),

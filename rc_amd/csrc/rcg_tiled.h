@@ -40,6 +40,7 @@ struct TiledDev {
   const int *blk_off;  // n_inst + 1: first binning workgroup of each instance
   int4 *pairs;       // (point, tile) records sorted by tile
   float2 *partial;   // degridding: per (point, tile slot) partial sums
+  int adj;           // gridding with conjugated taps (set per launch)
 };
 
 // Event pairs kept per direction for rcg_tiled_kernel_times.

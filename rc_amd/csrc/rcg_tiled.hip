@@ -577,7 +577,10 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
         more = kn < npw;
         load_group(tn, pn, sn, rsrc);
         __builtin_amdgcn_sched_barrier(0);
-        const float a = vx, b = p.conj ? vy : -vy, c = p.conj ? -vx : vx, e = vy;
+        // taps are conjugated for mirrored (w < 0) layers and, in the
+        // adjoint, once more for all of them
+        const bool cj = p.conj != d.adj;
+        const float a = vx, b = cj ? vy : -vy, c = cj ? -vx : vx, e = vy;
         int li[GRP];
         float2 x[GRP];
 #pragma unroll
@@ -1066,9 +1069,10 @@ extern "C" rcg_status rcg_tiled_bin(rcg_tiled_plan *pl, const rcg_gcf *gcf, void
   return RCG_OK;
 }
 
-extern "C" rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
-                                         const rcg_c64 *vis, rcg_c64 *grid,
-                                         void *stream)
+namespace {
+
+rcg_status grid_tiled(rcg_tiled_plan *pl, const rcg_gcf *gcf, const rcg_c64 *vis, rcg_c64 *grid,
+                      int adjoint, void *stream)
 {
   clear_error();
   if (!pl || !gcf || !vis || !grid) {
@@ -1078,7 +1082,8 @@ extern "C" rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
   rcg_status s = check_tap_table(gcf, "rcg_grid_tiled_f32");
   if (s != RCG_OK) return s;
   hipStream_t st = (hipStream_t)stream;
-  const TiledDev &d = pl->dev;
+  TiledDev d = pl->dev;
+  d.adj = adjoint;
   hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st,
                      reinterpret_cast<float *>(grid), d.gstride * d.n_inst * 2);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue, 0, 8LL);
@@ -1102,6 +1107,20 @@ extern "C" rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
     pl->n_ev[0]++;
   }
   return RCG_OK;
+}
+
+}  // namespace
+
+extern "C" rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
+                                         const rcg_c64 *vis, rcg_c64 *grid, void *stream)
+{
+  return grid_tiled(pl, gcf, vis, grid, 0, stream);
+}
+
+extern "C" rcg_status rcg_grid_tiled_adjoint_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
+                                                 const rcg_c64 *vis, rcg_c64 *grid, void *stream)
+{
+  return grid_tiled(pl, gcf, vis, grid, 1, stream);
 }
 
 extern "C" rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,

@@ -200,7 +200,9 @@ class TiledPlan:
             raise ValueError(f"grid must be a contiguous {self.grid_shape} tensor")
 
     def grid(self, gcf: DeviceGCF, vis: torch.Tensor, grid: torch.Tensor | None = None,
-             stream=None) -> torch.Tensor:
+             stream=None, adjoint: bool = False) -> torch.Tensor:
+        """Gridding; adjoint=True conjugates every tap, which makes it the
+        Hermitian adjoint of degrid()."""
         if gcf.data.dtype != torch.complex64:
             raise TypeError("FP32 path takes complex64 taps")
         self._check_vis(vis)
@@ -208,8 +210,8 @@ class TiledPlan:
             grid = torch.empty(self.grid_shape, dtype=torch.complex64, device=vis.device)
         self._check_grid(grid)
         t = gcf.struct()
-        _lib.check(_lib.lib().rcg_grid_tiled_f32(self._h, C.byref(t), vis.data_ptr(),
-                                                 grid.data_ptr(), _stream(stream)),
+        fn = _lib.lib().rcg_grid_tiled_adjoint_f32 if adjoint else _lib.lib().rcg_grid_tiled_f32
+        _lib.check(fn(self._h, C.byref(t), vis.data_ptr(), grid.data_ptr(), _stream(stream)),
                    "rcg_grid_tiled_f32")
         return grid
 
@@ -291,3 +293,61 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+class FFTPlan:
+    """rcg_fft plan: the reference's fft_inplace_even transform (sign -1/+1
+    C2C, 0 R2C, 2 C2R, fftshift before and after, unnormalised) on a device
+    buffer of `size` rows of `pitch` complex elements."""
+
+    def __init__(self, sign: int, size: int, pitch: int | None = None, f64: bool = False):
+        h = C.c_void_p()
+        self.sign, self.size, self.pitch, self.f64 = sign, size, pitch or size, f64
+        _lib.check(_lib.lib().rcg_fft_plan_create(sign, int(f64), size, self.pitch, C.byref(h)),
+                   "rcg_fft_plan_create")
+        self._h = h
+
+    def __call__(self, data: torch.Tensor, stream=None) -> torch.Tensor:
+        want = torch.complex128 if self.f64 else torch.complex64
+        if data.dtype != want or not data.is_contiguous() or \
+                data.numel() < self.size * self.pitch:
+            raise ValueError(f"FFT buffer must be contiguous {want} of size*pitch elements")
+        _lib.check(_lib.lib().rcg_fft_execute(self._h, data.data_ptr(), _stream(stream)),
+                   "rcg_fft_execute")
+        return data
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().rcg_fft_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hogbom(res: torch.Tensor, psf: torch.Tensor, niters: int, gain: float, threshold: float,
+           model: torch.Tensor | None = None, stream=None) -> tuple[torch.Tensor, int]:
+    """Hogbom CLEAN on device images (rcg_hogbom): res (float32/float64
+    [siz, pitch]) is cleaned in place; returns (model, accepted peaks)."""
+    if res.dtype != psf.dtype or res.shape != psf.shape or res.dim() != 2 or \
+            res.dtype not in (torch.float32, torch.float64):
+        raise ValueError("res and psf must be 2-D float tensors of one shape and dtype")
+    if not (res.is_contiguous() and psf.is_contiguous()):
+        raise ValueError("res and psf must be contiguous")
+    siz, pitch = res.shape
+    if model is None:
+        model = torch.empty_like(res)
+    h = _lib.lib()
+    scratch = torch.empty(int(h.rcg_hogbom_scratch_bytes()), dtype=torch.uint8,
+                          device=res.device)
+    s = _stream(stream)
+    _lib.check(h.rcg_hogbom(model.data_ptr(), res.data_ptr(), psf.data_ptr(),
+                            int(res.dtype == torch.float64), siz, pitch, niters, gain,
+                            threshold, scratch.data_ptr(), s), "rcg_hogbom")
+    it = C.c_uint()
+    _lib.check(h.rcg_hogbom_iterations(scratch.data_ptr(), C.byref(it), s),
+               "rcg_hogbom_iterations")
+    return model, int(it.value)
