@@ -43,11 +43,14 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", choices=["wproj", "batched", "imaging"], default="wproj",
+    ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision"],
+                    default="wproj",
                     help="wproj: the 4096^2 w-projection workload (default); batched: 64 "
                          "uneven 2048^2 snapshot instances, one graph launch vs one launch "
                          "per instance; imaging: the differentiable measurement operator "
-                         "on a 2048^2 sky, 20 gradient steps and 1000 Hogbom iterations")
+                         "on a 2048^2 sky, 20 gradient steps and 1000 Hogbom iterations; "
+                         "precision: 50M points of the w-projection layout, FP32 with and "
+                         "without compensated flushes against the FP64 path")
     ap.add_argument("--instances", type=int, default=64)
     return ap.parse_args()
 
@@ -311,6 +314,87 @@ def bench_imaging(args) -> int:
     return 0
 
 
+def bench_precision(args) -> int:
+    """The w-projection layout at 1530 time samples (49,939,200 points):
+    FP32 gridding with plain and with compensated (TwoSum) tile flushes, and
+    FP32 degridding, against the FP64 GPU path on the same complex64 inputs
+    (the FP64 path itself is within 1e-12 of the reference's CPU gridder).
+    Reports relative L2 and max-abs errors, kernel times, algorithmic HBM
+    GB/s and FP32 VALU utilisation."""
+    import torch
+    from rc_amd import device as D
+    from rc_amd import synth
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    vc, gcf, N = synth.wproj(n_times=args.n_times if args.n_times != 306 else 1530)
+    log(f"precision workload: {vc.n} points, {time.time() - t0:.1f} s to generate")
+    geom = D.DeviceGeom.from_chunks(vc, N)
+    dg32 = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
+    dg64 = D.DeviceGCF.from_table(gcf, dtype=np.complex128)
+    vis32 = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).to(dev)
+    plan = D.TiledPlan(geom, tile=args.tile, chunk=args.chunk, max_support=64)
+    plan.bin(dg32)
+    taps = plan.stats()["taps"]
+    # FP64 references on the same inputs
+    pg, _, _ = D.pregrid(geom, dg64)
+    g64 = D.grid_plain_f64(geom, dg64, pg, vis32.to(torch.complex128))
+    torch.cuda.synchronize()
+
+    def errs(a, b):
+        d = (a.to(torch.complex128) - b)
+        return {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(b)),
+                "max_abs": float(d.abs().max()), "max_ref": float(b.abs().max())}
+
+    def kernel_ms(which, fn, reps=5):
+        fn()
+        plan.timing(True)
+        for _ in range(reps):
+            out = fn()
+        t = float(np.mean(plan.kernel_times(which)))
+        plan.timing(False)
+        return t, out
+
+    b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
+    res = {}
+    for comp in (False, True):
+        plan.compensated(comp)
+        tk, g32 = kernel_ms("grid", lambda: plan.grid(dg32, vis32))
+        torch.cuda.synchronize()
+        res["grid_compensated" if comp else "grid_plain"] = {
+            **errs(g32, g64), "kernel_ms": round(tk, 4),
+            "hbm_gbs_algorithmic": round(b_dir / (tk / 1e3) / 1e9, 1),
+            "valu_frac": round(8 * taps / (tk / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)}
+        del g32
+    plan.compensated(False)
+    g_in = g64.to(torch.complex64)
+    td, v32 = kernel_ms("degrid", lambda: plan.degrid(dg32, g_in))
+    v64 = D.degrid_plain_f64(geom, dg64, pg, g_in.to(torch.complex128))
+    res["degrid"] = {**errs(v32, v64), "kernel_ms": round(td, 4),
+                     "hbm_gbs_algorithmic": round(b_dir / (td / 1e3) / 1e9, 1),
+                     "valu_frac": round(8 * taps / (td / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)}
+    torch.cuda.synchronize()
+    log(json.dumps(res))
+    gp, gc = res["grid_plain"], res["grid_compensated"]
+    value = vc.n / ((gp["kernel_ms"] + td) / 1e3) / 1e6
+    line = {
+        "metric": "precision study: FP32 grid/degrid error vs FP64 at 50M points, 4096² 32 "
+                  "w-planes (Mvis/s of the plain-flush kernels)",
+        "value": round(value, 3), "unit": "Mvis/s", "n_gpus": 1, "steps": 5, "warmup": 1,
+        "ms_per_step": round(gp["kernel_ms"] + td, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64/f32 (f64 pregrid)",
+        "data": "synthetic",
+        "config": {"workload": "w-projection layout, 256 antennas x 1530 times, 4096x4096, "
+                               "32 w-planes, supports 8-64", "points": vc.n,
+                   "taps_per_direction": taps, "tile": args.tile, "chunk": args.chunk},
+        "errors_vs_fp64_gpu": res,
+        "compensation_gain": round(gp["rel_l2"] / max(gc["rel_l2"], 1e-30), 3),
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -329,6 +413,8 @@ def main() -> int:
         return bench_batched(args)
     if args.workload == "imaging":
         return bench_imaging(args)
+    if args.workload == "precision":
+        return bench_precision(args)
     dev = torch.device("cuda:0")
     t0 = time.time()
     vc, gcf, N = synth.wproj(n_times=args.n_times)

@@ -239,6 +239,13 @@ rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
 rcg_status rcg_grid_tiled_adjoint_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                                       const rcg_c64 *vis, rcg_c64 *grid,
                                       void *stream);
+/* Compensated flush (off by default): gridding adds each tile's sum to the
+ * grid with atomics whose rounding errors (exact, by TwoSum on the value
+ * the atomic returns) are accumulated in a second grid and added at the
+ * end, so the error of the flushes drops to second order.  Enabling it
+ * allocates that grid once (not capturable); the switch itself may change
+ * between calls. */
+rcg_status rcg_tiled_compensated(rcg_tiled_plan *plan, int enable);
 /* Synchronises `stream`; returns the in-bound tap count, pair and work-item
  * counts and the device error word of the last rcg_tiled_bin (bit 0: a point
  * referenced a layer outside the table; bit 1: pair buffer overflow). */

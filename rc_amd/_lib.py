@@ -73,6 +73,7 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_tiled_plan_destroy": (C.c_int, [vp]),
     "rcg_tiled_bin": (C.c_int, [vp, C.POINTER(Gcf), vp]),
     "rcg_tiled_timing": (C.c_int, [vp, C.c_int]),
+    "rcg_tiled_compensated": (C.c_int, [vp, C.c_int]),
     "rcg_tiled_kernel_times": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int,
                                          C.POINTER(C.c_int)]),
     "rcg_tiled_graph_create": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp, C.c_int,

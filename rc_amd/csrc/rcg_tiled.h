@@ -41,6 +41,7 @@ struct TiledDev {
   int4 *pairs;       // (point, tile) records sorted by tile
   float2 *partial;   // degridding: per (point, tile slot) partial sums
   int adj;           // gridding with conjugated taps (set per launch)
+  float2 *lo;        // compensated flush: rounding errors of the atomic adds
 };
 
 // Event pairs kept per direction for rcg_tiled_kernel_times.
@@ -61,6 +62,8 @@ struct rcg_tiled_plan {
   unsigned long long *taps = nullptr;
   int *err = nullptr;
   int *offs = nullptr;    // device pt_off | blk_off
+  float2 *lo = nullptr;   // compensated flush grid (rcg_tiled_compensated)
+  bool compensated = false;
   rcg::TiledDev dev{};
   // kernel timing (rcg_tiled_timing)
   bool timing = false;

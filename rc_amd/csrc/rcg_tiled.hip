@@ -512,6 +512,17 @@ __device__ __forceinline__ void stream_batch(const TiledDev &d, const float2 *__
   }
 }
 
+// Error of the rounded sum fl(a + b) (Knuth's TwoSum): fl(a + b) + err =
+// a + b exactly, in round-to-nearest without overflow.
+__device__ __forceinline__ float two_sum_err(float a, float b)
+{
+#pragma clang fp contract(off)
+  const float s = a + b;
+  const float bp = s - a;
+  const float ap = s - bp;
+  return (a - ap) + (b - bp);
+}
+
 // ---------------------------------------------------------------------------
 // gridding: each wavefront owns a private copy of the tile in LDS, so the
 // read-modify-write of a cell needs no atomic: within one pair every lane
@@ -636,11 +647,38 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       if (ir.single) {
         *g = make_float2(re, im);
       } else if (re != 0.f || im != 0.f) {
-        unsafeAtomicAdd(&g->x, re);
-        unsafeAtomicAdd(&g->y, im);
+        if (d.lo) {
+          // compensated flush: the exact rounding error of each atomic
+          // add (old + x, as the atomic computed it) goes to a second grid
+          const float ox = unsafeAtomicAdd(&g->x, re);
+          const float oy = unsafeAtomicAdd(&g->y, im);
+          float2 *l = d.lo + ir.goff + (long long)gu * d.pitch + gv;
+          const float ex = two_sum_err(ox, re), ey = two_sum_err(oy, im);
+          if (ex != 0.f) unsafeAtomicAdd(&l->x, ex);
+          if (ey != 0.f) unsafeAtomicAdd(&l->y, ey);
+        } else {
+          unsafeAtomicAdd(&g->x, re);
+          unsafeAtomicAdd(&g->y, im);
+        }
       }
     }
   });
+}
+
+// grid += lo, cell by cell (the end of a compensated gridding)
+__global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restrict__ lo,
+                               long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 l = lo[i];
+    if (l.x != 0.f || l.y != 0.f) {
+      float2 g = grid[i];
+      g.x += l.x;
+      g.y += l.y;
+      grid[i] = g;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1021,7 +1059,7 @@ extern "C" rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *pl)
   TiledDev &d = pl->dev;
   void *ptrs[] = {pl->geoms_dev, pl->offs, pl->pg, d.tile_count, d.tile_start,
                   d.tile_cursor, d.item_start, d.totals, d.seg, d.queue, pl->taps, pl->err,
-                  d.pairs, d.partial};
+                  d.pairs, d.partial, pl->lo};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int dir = 0; dir < 2; dir++)
@@ -1084,8 +1122,12 @@ rcg_status grid_tiled(rcg_tiled_plan *pl, const rcg_gcf *gcf, const rcg_c64 *vis
   hipStream_t st = (hipStream_t)stream;
   TiledDev d = pl->dev;
   d.adj = adjoint;
+  d.lo = pl->compensated ? pl->lo : nullptr;
   hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st,
                      reinterpret_cast<float *>(grid), d.gstride * d.n_inst * 2);
+  if (d.lo)
+    hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st,
+                       reinterpret_cast<float *>(d.lo), d.gstride * d.n_inst * 2);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue, 0, 8LL);
   RCG_HIP_TRY(hipGetLastError());
   if (pl->n == 0) return RCG_OK;
@@ -1106,10 +1148,34 @@ rcg_status grid_tiled(rcg_tiled_plan *pl, const rcg_gcf *gcf, const rcg_c64 *vis
     RCG_HIP_TRY(hipEventRecord(ev[1], st));
     pl->n_ev[0]++;
   }
+  if (d.lo) {
+    hipLaunchKernelGGL(k_comp_combine, dim3(2048), dim3(256), 0, st, g, d.lo,
+                       d.gstride * d.n_inst);
+    RCG_HIP_TRY(hipGetLastError());
+  }
   return RCG_OK;
 }
 
 }  // namespace
+
+extern "C" rcg_status rcg_tiled_compensated(rcg_tiled_plan *pl, int enable)
+{
+  clear_error();
+  if (!pl) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_compensated: null plan");
+    return RCG_ERR_ARG;
+  }
+  if (enable && !pl->lo) {
+    const size_t cells = (size_t)pl->dev.gstride * pl->dev.n_inst;
+    if (hipMalloc(reinterpret_cast<void **>(&pl->lo), cells * sizeof(float2)) != hipSuccess) {
+      pl->lo = nullptr;
+      set_error(RCG_ERR_NOMEM, "rcg_tiled_compensated: %zu cells", cells);
+      return RCG_ERR_NOMEM;
+    }
+  }
+  pl->compensated = enable != 0;
+  return RCG_OK;
+}
 
 extern "C" rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gcf,
                                          const rcg_c64 *vis, rcg_c64 *grid, void *stream)

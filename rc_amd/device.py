@@ -240,6 +240,11 @@ class TiledPlan:
         return {"taps": taps.value, "pairs": pairs.value, "items": items.value,
                 "err": err.value}
 
+    def compensated(self, enable: bool) -> None:
+        """Compensated (TwoSum) tile flushes for gridding."""
+        _lib.check(_lib.lib().rcg_tiled_compensated(self._h, int(enable)),
+                   "rcg_tiled_compensated")
+
     def timing(self, enable: bool) -> None:
         _lib.check(_lib.lib().rcg_tiled_timing(self._h, int(enable)), "rcg_tiled_timing")
 

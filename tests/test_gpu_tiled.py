@@ -82,3 +82,28 @@ def test_tiled_support_above_plan_is_reported(hip_device):
     plan.bin(dg)
     assert plan.stats()["err"] & 2
     plan.close()
+
+
+def test_compensated_flush_vs_oracle(hip_device):
+    """Small work items (many flushes per tile) on the 4096^2 / 32-plane
+    slice: the compensated flush stays within the bar and is at least as
+    close to the FP64 oracle as the plain flush."""
+    vc, gcf, N = synth.wproj(n_ant=64, n_times=64, vis_dtype=np.complex128)
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N, nthreads=8)
+    geom = D.DeviceGeom.from_chunks(vc, N)
+    dg = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
+    plan = D.TiledPlan(geom, tile=32, chunk=8)
+    plan.bin(dg)
+    vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).cuda()
+    plain = plan.grid(dg, vis).cpu().numpy()
+    plan.compensated(True)
+    comp = plan.grid(dg, vis).cpu().numpy()
+    plan.compensated(False)
+    again = plan.grid(dg, vis).cpu().numpy()
+    e_plain, e_comp = rel_l2(plain, ref), rel_l2(comp, ref)
+    assert plan.stats()["taps"] == taps
+    assert e_comp <= TOL and e_plain <= TOL
+    assert e_comp <= e_plain
+    assert rel_l2(again, ref) <= TOL
+    plan.close()
