@@ -127,6 +127,26 @@ void rcg_fft_destroy_plan(void *plan);
 void deconvolve(double *mod_p, double *res_p, double *psf_p, int siz,
                 int pitch, unsigned int niters, double gain, double threshold);
 
+/* Replaces mkGCFLayer (MS4/kernel/cpu/gcf/GCF.cpp:123-146, FFI import
+ * MS4/dna-programs/Kernel/CPU/GCF.hs:15-25): the 64 oversampled layers
+ * [over_u][over_v][support][support] of one w-plane for phase w (scaled)
+ * and theta/2 = t2 into host dst, table[i] = dst + i*support^2.  p is the
+ * handle returned by the previous call (NULL the first time), reused while
+ * max_support and src_pad stay the same; `arena` is not used (the work
+ * buffer is on the device).  Release with rcg_gcf_gen_destroy.  The
+ * normalisation sums are pad-correct unless rcg_gcf_pad_quirk(1) selects
+ * the reference's contiguous walk (thread-local switch). */
+void *mkGCFLayer(void *p, rcg_c128 dst[], rcg_c128 *table[], rcg_c128 arena[],
+                 int support, int max_support, int src_pad, double t2,
+                 double w);
+void rcg_gcf_pad_quirk(int enable);
+/* Replaces calcAccums (GCF.cpp:148-219): per w-plane sum of w and point
+ * count over numDataPoints points, plane = round(w / wstep) + numOfWPlanes/2.
+ * Points outside the planes are skipped and reported (RCG_ERR_RANGE); the
+ * reference writes outside its arrays for them. */
+void calcAccums(const rcg_double3 uvw[], double sums[], int npts[],
+                double wstep, int numDataPoints, int numOfWPlanes);
+
 /* Thread-local error of the last call made on this thread (0 = none). */
 int rcg_last_error(void);
 const char *rcg_last_error_string(void);
@@ -301,6 +321,22 @@ rcg_status rcg_fft_plan_create(int sign, int f64, int size, int pitch,
                                rcg_fft_plan **plan);
 rcg_status rcg_fft_execute(rcg_fft_plan *plan, void *data, void *stream);
 rcg_status rcg_fft_plan_destroy(rcg_fft_plan *plan);
+
+/* GCF generation on device memory (the reference's mkGCFLayer and
+ * calcAccums, GCF.cpp:16-219).  A generator owns the (8 max_support)^2
+ * FP64 arena with row pitch 8 max_support + src_pad and its rocFFT plan;
+ * rcg_gcf_gen_layer writes one plane's 64 layers [8][8][support][support]
+ * to dst (complex128 if f64, else complex64).  rcg_calc_accums: device uvw,
+ * sums (double) and npts (int) of n_planes entries, err_dev one device int
+ * (bit 0: points outside the planes, skipped). */
+typedef struct rcg_gcf_gen rcg_gcf_gen;
+rcg_status rcg_gcf_gen_create(int max_support, int src_pad, rcg_gcf_gen **gen);
+rcg_status rcg_gcf_gen_layer(rcg_gcf_gen *gen, void *dst, int f64, int support,
+                             double t2, double w, int pad_quirk, void *stream);
+rcg_status rcg_gcf_gen_destroy(rcg_gcf_gen *gen);
+rcg_status rcg_calc_accums(const rcg_double3 *uvw, long long n, double wstep,
+                           int n_planes, double *sums, int *npts, int *err_dev,
+                           void *stream);
 
 /* Hogbom CLEAN on device images (size siz x siz, row pitch `pitch`
  * elements, double if f64 else float).  scratch: rcg_hogbom_scratch_bytes()

@@ -109,6 +109,15 @@ SIGNATURES: dict[str, tuple] = {
     "fftInitThreading": (None, []),
     "rcg_fft_destroy_plan": (None, [vp]),
     "deconvolve": (None, [vp, vp, vp, C.c_int, C.c_int, C.c_uint, C.c_double, C.c_double]),
+    "mkGCFLayer": (vp, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "rcg_gcf_pad_quirk": (None, [C.c_int]),
+    "calcAccums": (None, [vp, vp, vp, C.c_double, C.c_int, C.c_int]),
+    # GCF generation (device)
+    "rcg_gcf_gen_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
+    "rcg_gcf_gen_layer": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                    vp]),
+    "rcg_gcf_gen_destroy": (C.c_int, [vp]),
+    "rcg_calc_accums": (C.c_int, [vp, C.c_longlong, C.c_double, C.c_int, vp, vp, vp, vp]),
 }
 
 
