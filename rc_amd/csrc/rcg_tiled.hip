@@ -427,9 +427,12 @@ __device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned vo
 //
 // Latency: a wavefront loads the records (and visibilities) of 64 pairs at
 // once, one per lane, double-buffered one batch ahead, and broadcasts them
-// with readlane; the taps of a pair's first group of up to four slabs are
-// loaded while the previous pair's LDS updates run.
-constexpr int GRP = 4;
+// with readlane; the taps of a pair's first group of up to GRP slabs are
+// loaded while the previous pair's LDS updates run.  Three slabs per group
+// balance the loads kept in flight against the slots a short pair leaves
+// empty (measured on the 4096^2 w-projection workload: 2 and 4 slabs are
+// 4-10% slower, 1 and 6 slabs 17-20% slower).
+constexpr int GRP = 3;
 
 struct Pair {
   int h, rows, nslabs, point, whole, conj, slot;
