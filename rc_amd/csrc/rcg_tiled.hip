@@ -462,10 +462,11 @@ __device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int 
   // lane / w through the approximate reciprocal: (lane + 0.5) / w is at
   // least 0.5 / w away from an integer, far above its rounding error
   const int lr = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)w));
-  const int lc = lane - lr * w;
+  // 24-bit multiplies (full rate): lr < 64, w <= 64, stride < 2^16
+  const int lc = lane - __mul24(lr, w);
   const bool idle = lr >= p.rows;
   p.lr = idle ? 64 : lr;
-  p.tb0 = idle ? TAP_OOB : (unsigned)(py + lr * stride + lc) * 8u;
+  p.tb0 = idle ? TAP_OOB : (unsigned)(py + __mul24(lr, stride) + lc) * 8u;
   p.li0 = idle ? DUMMY : (r0 + lr) * LS + c0 + lc;
   p.tstep = p.rows * stride * 8;
   p.lstep = p.rows * LS;
