@@ -34,6 +34,13 @@
  *                        force on tiny images (single point source recovered
  *                        exactly, residual = dirty - model * psf).
  *
+ * Strength of the pinning: tests/test_oracle_mutations.py plants ten
+ * plausible mistakes here (orc_set_mutation: transposed tap operand,
+ * dropped conjugation, rounded oversampling offset, dropped recentring,
+ * conjugating degridder, non-strict in-bound test, scaled w-plane, wrong
+ * normalisation, flipped rotation sign, signed Hogbom peak) and requires
+ * each to fail at least one of the checks above.
+ *
  * Build: plain C11, no fast-math, no FP contraction, so that the operation
  * order below is the one executed (oracle/Makefile).
  */
@@ -46,6 +53,21 @@
 #endif
 
 #define ORC_OVER 8 /* MS4/kernel/common/metrix.h:5 (OVER) */
+
+/* Planted mistakes, for tests that show the pinning checks catch them
+ * (tests/test_oracle_mutations.py).  0 = the restatement as written.
+ *   1 transposed tap operand (layer[sv][su])
+ *   2 HalfGCF conjugation dropped
+ *   3 oversampling offset rounded instead of floored
+ *   4 window recentring dropped
+ *   5 degridder conjugates its taps (adjoint instead of transpose)
+ *   6 in-bound test not strict (u0 <= N - S accepted)
+ *   7 w-plane from the scaled w
+ *   8 normalisation by grid_size^2 instead of grid_size*grid_pitch
+ *   9 phase rotation with the opposite sign
+ *  10 Hogbom peak by signed value instead of magnitude */
+static int orc_mutation = 0;
+void orc_set_mutation(int m) { orc_mutation = m; }
 
 typedef struct { double re, im; } ocd;       /* complexd, CH:37-39 */
 typedef struct { double u, v, w; } od3;      /* Double3,  CH:72-77 */
@@ -71,7 +93,7 @@ void orc_pregrid(double scale, double wstep, const od3 *p, int max_supp,
 {
   double us = p->u * scale;
   double vs = p->v * scale;
-  double wr = round(p->w / wstep);
+  double wr = round(orc_mutation == 7 ? p->w * scale / wstep : p->w / wstep);
   double fu = floor(us), fv = floor(vs);
   r->valid = 1;
   if (!(fu >= -32768.0 && fu <= 32767.0) || !(fv >= -32768.0 && fv <= 32767.0)
@@ -84,6 +106,10 @@ void orc_pregrid(double scale, double wstep, const od3 *p, int max_supp,
   /* CH:130-131: over * (uvw.u - u) with u the int16 floor */
   int over_u = (int)floor(ORC_OVER * (us - (double)u));
   int over_v = (int)floor(ORC_OVER * (vs - (double)v));
+  if (orc_mutation == 3) {
+    over_u = (int)round(ORC_OVER * (us - (double)u)) % ORC_OVER;
+    over_v = (int)round(ORC_OVER * (vs - (double)v)) % ORC_OVER;
+  }
   /* CH:136-137 */
   u += grid_size / 2 - max_supp / 2;
   v += grid_size / 2 - max_supp / 2;
@@ -134,10 +160,13 @@ static void orc_resolve(const orc_pg *pg, int half, const ocd *const *gcf,
   s->v = pg->v;
   s->inb = pg->valid && !(pg->u < 0 || pg->u >= grid_size - max_supp ||
                           pg->v < 0 || pg->v >= grid_size - max_supp);
+  if (orc_mutation == 6)
+    s->inb = pg->valid && !(pg->u < 0 || pg->u > grid_size - max_supp ||
+                            pg->v < 0 || pg->v > grid_size - max_supp);
   if (!pg->valid) { s->layer = NULL; s->stride = 0; s->conj = 0; return; }
   /* SG:130-137 */
   if (half) {
-    s->conj = idx < 0;
+    s->conj = orc_mutation == 2 ? 0 : idx < 0;
     s->layer = gcf[idx < 0 ? -idx : idx];
   } else {
     s->conj = 0;
@@ -146,12 +175,13 @@ static void orc_resolve(const orc_pg *pg, int half, const ocd *const *gcf,
   /* SG:139-140: gcf_supps indexed by the signed plane in both variants */
   int gs = gcf_supps[pg->w_plane];
   s->stride = gs;
-  s->layer += (gs - max_supp) / 2 * (gs + 1);
+  if (orc_mutation != 4) s->layer += (gs - max_supp) / 2 * (gs + 1);
 }
 
 static inline ocd orc_tap(const orc_src *s, int su, int sv)
 {
-  ocd t = s->layer[su * s->stride + sv];     /* SG:158 */
+  ocd t = orc_mutation == 1 ? s->layer[sv * s->stride + su]
+                            : s->layer[su * s->stride + sv];     /* SG:158 */
   if (s->conj) t.im = -t.im;                 /* SG:160-161 */
   return t;
 }
@@ -263,6 +293,7 @@ unsigned long long orc_degrid(int half, double scale, double wstep,
           const ocd *grow = grid + (size_t)(sa[i].u + su) * grid_pitch + sa[i].v;
           for (int sv = 0; sv < ms; sv++) {
             ocd t = orc_tap(sa + i, su, sv);
+            if (orc_mutation == 5) t.im = -t.im;
             ocd g = grow[sv];
             /* SG:171: vis[i] += grid * supportPixel */
             double pr = g.re * t.re - g.im * t.im;
@@ -285,7 +316,7 @@ unsigned long long orc_degrid(int half, double scale, double wstep,
 void orc_normalize(ocd *src, int grid_pitch, int grid_size)
 {
   size_t siz = (size_t)grid_size * (size_t)grid_pitch;
-  double norm = 1.0 / (double)(grid_size * grid_pitch);
+  double norm = 1.0 / (double)(grid_size * (orc_mutation == 8 ? grid_size : grid_pitch));
   for (size_t i = 0; i < siz; i++) {
     src[i].re *= norm;
     src[i].im *= norm;
@@ -296,7 +327,7 @@ void orc_normalize(ocd *src, int grid_pitch, int grid_size)
 void orc_rotate(const od3 *uvw, ocd *vis, int n, double scale)
 {
   for (int i = 0; i < n; i++) {
-    double ph = 2.0 * M_PI * (uvw[i].w * scale);
+    double ph = (orc_mutation == 9 ? -2.0 : 2.0) * M_PI * (uvw[i].w * scale);
     double s = sin(ph), c = cos(ph);
     double re = vis[i].re * c - vis[i].im * s;
     double im = vis[i].re * s + vis[i].im * c;
@@ -415,7 +446,8 @@ unsigned orc_hogbom(double *mod, double *res, const double *psf, int siz,
     double best = -1.0;
     for (int i = 0; i < siz; i++)
       for (int j = 0; j < siz; j++) {
-        double a = fabs(res[(size_t)i * pitch + j]);
+        double a = orc_mutation == 10 ? res[(size_t)i * pitch + j]
+                                      : fabs(res[(size_t)i * pitch + j]);
         if (a > best) { best = a; rx = i; ry = j; }
       }
     double v = res[(size_t)rx * pitch + ry];

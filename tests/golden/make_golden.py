@@ -100,6 +100,16 @@ def make_probe(out: Path) -> None:
             pts = probe_points(rng, 120, N, scale, wstep, max_plane)
             # chunk support: the minimum plane support (no over-read)
             S_chunk = rng.choice([3, 4, 5, 6], len(pts))
+            # every sixth point sits on an edge of the in-bound range of its
+            # support: top-left pixel N - S - 1, N - S, 0 or -1 (u, then v)
+            for k in range(5, len(pts), 6):
+                S = int(S_chunk[k])
+                target = (N - S - 1, N - S, 0, -1)[(k // 6) % 4]
+                pix = target - N // 2 + S // 2 + 0.3
+                if (k // 24) % 2 == 0:
+                    pts[k, 0] = pix / scale
+                else:
+                    pts[k, 1] = pix / scale
             res = []
             for (u, v, w), S in zip(pts, S_chunk):
                 vc = VisChunks(uvw=np.array([[[u, v, w]]]), vis=np.ones((1, 1), np.complex128),
