@@ -310,6 +310,15 @@ rcg_status rcg_grid0(const rcg_double3 *uvw, const rcg_c128 *vis,
 rcg_status rcg_reweight(const rcg_double3 *uvw, rcg_c128 *vis, double scale,
                         int n, int grid_size, int *count_scratch, void *stream);
 rcg_status rcg_herm_padded(rcg_c128 *data, int size, int pitch, void *stream);
+/* herm_padded_inplace on complex128 (f64) or complex64 data. */
+rcg_status rcg_herm(void *data, int f64, int size, int pitch, void *stream);
+/* Major-cycle helpers: out -= in over n complex values (the reference's
+ * subtractVis, MS4/kernel/gpu/nvidia_degrid/nvidia_degrid.cu:11-18), and
+ * vis = re + i im (constVis, MS4/dna-programs/Data.hs:242-247). */
+rcg_status rcg_subtract_vis(void *out, const void *in, int f64, long long n,
+                            void *stream);
+rcg_status rcg_const_vis(void *vis, int f64, long long n, double re, double im,
+                         void *stream);
 rcg_status rcg_fftshift_even(void *data, int elem_bytes, int size, int pitch,
                              void *stream);
 

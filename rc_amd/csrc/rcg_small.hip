@@ -11,6 +11,8 @@
 #include <cstring>
 #include <vector>
 
+#include <algorithm>
+
 #include "rcg_internal.h"
 
 namespace rcg {
@@ -104,15 +106,45 @@ __global__ void k_reweight_apply(const rcg_double3 *__restrict__ uvw, double2 *_
 
 // A[i][j] += conj(A[size-1-i][size-1-j]) for j < size/2
 // (herm_padded.cpp:6-27, default branch); sources are never written.
-__global__ void k_herm(double2 *__restrict__ d, int size, int pitch)
+template <class C>
+__global__ void k_herm(C *__restrict__ d, int size, int pitch)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= size / 2 || i >= size) return;
-  const double2 s = d[(long long)(size - 1 - i) * pitch + (size - 1 - j)];
-  double2 &t = d[(long long)i * pitch + j];
+  const C s = d[(long long)(size - 1 - i) * pitch + (size - 1 - j)];
+  C &t = d[(long long)i * pitch + j];
   t.x += s.x;
   t.y -= s.y;
+}
+
+// out -= in (MS4/kernel/gpu/nvidia_degrid/nvidia_degrid.cu:11-18, the
+// major cycle's subtraction of predicted from observed visibilities).
+template <class C>
+__global__ void k_subtract(C *__restrict__ out, const C *__restrict__ in, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    C o = out[i];
+    const C a = in[i];
+    o.x -= a.x;
+    o.y -= a.y;
+    out[i] = o;
+  }
+}
+
+// vis = value (constVis, MS4/dna-programs/Data.hs:242-247: the PSF's
+// visibilities are all 1).
+template <class C, class E>
+__global__ void k_const(C *__restrict__ v, long long n, E re, E im)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    C c;
+    c.x = re;
+    c.y = im;
+    v[i] = c;
+  }
 }
 
 // Quadrant swap of an even size x size array, row pitch in elements
@@ -279,8 +311,66 @@ extern "C" rcg_status rcg_herm_padded(rcg_c128 *data, int size, int pitch, void 
     if (!data || (size & 1)) set_error(RCG_ERR_ARG, "rcg_herm_padded: null data or odd size");
     return RCG_ERR_ARG;
   }
-  hipLaunchKernelGGL(k_herm, dim3((size / 2 + 255) / 256, size), dim3(256), 0,
+  hipLaunchKernelGGL(k_herm<double2>, dim3((size / 2 + 255) / 256, size), dim3(256), 0,
                      (hipStream_t)stream, reinterpret_cast<double2 *>(data), size, pitch);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_herm(void *data, int f64, int size, int pitch, void *stream)
+{
+  clear_error();
+  if (!data || size <= 0 || pitch < size) {
+    set_error(RCG_ERR_ARG, "rcg_herm: bad arguments");
+    return RCG_ERR_ARG;
+  }
+  const dim3 g((size / 2 + 255) / 256, size);
+  if (f64)
+    hipLaunchKernelGGL(k_herm<double2>, g, dim3(256), 0, (hipStream_t)stream,
+                       static_cast<double2 *>(data), size, pitch);
+  else
+    hipLaunchKernelGGL(k_herm<float2>, g, dim3(256), 0, (hipStream_t)stream,
+                       static_cast<float2 *>(data), size, pitch);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_subtract_vis(void *out, const void *in, int f64, long long n,
+                                       void *stream)
+{
+  clear_error();
+  if ((n > 0 && (!out || !in)) || n < 0) {
+    set_error(RCG_ERR_ARG, "rcg_subtract_vis: bad arguments");
+    return RCG_ERR_ARG;
+  }
+  if (n == 0) return RCG_OK;
+  const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+  if (f64)
+    hipLaunchKernelGGL(k_subtract<double2>, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<double2 *>(out), static_cast<const double2 *>(in), n);
+  else
+    hipLaunchKernelGGL(k_subtract<float2>, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<float2 *>(out), static_cast<const float2 *>(in), n);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_const_vis(void *vis, int f64, long long n, double re, double im,
+                                    void *stream)
+{
+  clear_error();
+  if ((n > 0 && !vis) || n < 0) {
+    set_error(RCG_ERR_ARG, "rcg_const_vis: bad arguments");
+    return RCG_ERR_ARG;
+  }
+  if (n == 0) return RCG_OK;
+  const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+  if (f64)
+    hipLaunchKernelGGL((k_const<double2, double>), dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<double2 *>(vis), n, re, im);
+  else
+    hipLaunchKernelGGL((k_const<float2, float>), dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<float2 *>(vis), n, (float)re, (float)im);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
