@@ -1,0 +1,84 @@
+/*
+ * rcgrid_halide.h -- the MS6 dna-flow kernel entry points (Halide buffer_t
+ * ABI) over librcgrid.so's HIP kernels.
+ *
+ * buffer_t has the layout of Halide's legacy buffer descriptor, which the
+ * MS6 kernels and their Haskell marshalling use
+ * (MS6/kernel/common/halide_buf.h:17-27,
+ * MS6/dna/flow/Flow/Halide/BufferT.hs:26-99).  Every entry point below runs
+ * on the GPU and leaves its result in host memory.
+ *
+ * Semantics (MS6/kernel/cpu/gridding/scatter.cpp:17-145 and
+ * degrid.cpp:17-124):
+ *   vis     doubles, extent {5, n}: (u, v, w, re, im) per point, point index
+ *           from min[1];
+ *   gcf     doubles, extent {2, S, S, 64}: one oversampled GCF of support S
+ *           (8, 16, 32 or 64), [over][y][x][re|im] with over = ou + 8 ov;
+ *   uvg     doubles, extent {2, X, Y} with min[1], min[2] the grid position
+ *           of the buffer's first cell: [y][x][re|im];
+ *   pixel   us = u*scale, ou = clamp(round(8 (us - floor us)), 0, 7),
+ *           x0 = round(us) + grid_size/2 - S/2 (round: halves to even, as
+ *           Halide's round), likewise v;
+ *   bounds  m = max(0, (margin_size - S)/2); a point is used only if
+ *           min[1] + m <= x0 <= min[1] + extent[1] - S - 1 - m (same in y);
+ *   scatter uvg(x0 + sx, y0 + sy) += vis * gcf[over][sy][sx] over every
+ *           point (the buffer is added to, not cleared);
+ *   degrid  vis_out = vis, then vis_out.re/im -= sum over the window of
+ *           uvg(x0 + sx, y0 + sy) * gcf[over][sy][sx].
+ * Return 0, or the reference's codes for an unsupported GCF size (-444
+ * scatter, -555 degrid, -222 / -333 for the _gpu1 variants); other
+ * failures return -1 with rcg_last_error() set.
+ */
+#ifndef RCGRID_HALIDE_H
+#define RCGRID_HALIDE_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef BUFFER_T_DEFINED
+#define BUFFER_T_DEFINED
+typedef struct buffer_t {
+  uint64_t dev;
+  uint8_t *host;
+  int32_t extent[4];
+  int32_t stride[4];
+  int32_t min[4];
+  int32_t elem_size;
+  __attribute__((aligned(1))) bool host_dirty;
+  __attribute__((aligned(1))) bool dev_dirty;
+  __attribute__((aligned(1))) uint8_t _padding[10 - sizeof(void *)];
+} buffer_t;
+#endif
+
+/* Replaces kern_scatter (MS6/kernel/cpu/gridding/scatter1.cpp:9-19). */
+int kern_scatter(const double scale, const int32_t grid_size,
+                 const int32_t margin_size, buffer_t *vis, buffer_t *gcf,
+                 buffer_t *uvg);
+/* Replaces kern_degrid (MS6/kernel/cpu/gridding/degrid1.cpp:9-18). */
+int kern_degrid(const double scale, const int32_t grid_size,
+                const int32_t margin_size, buffer_t *gcf, buffer_t *uvg,
+                buffer_t *vis, buffer_t *vis_out);
+/* Replace kern_scatter_gpu1 / kern_degrid_gpu1
+ * (MS6/kernel/gpu/gridding/kern_scatter_gpu1.cpp:22-40,
+ * kern_degrid_gpu1.cpp:21-41): a buffer whose `dev` is non-zero and whose
+ * host copy is not dirty is read from device memory (dev is a HIP device
+ * pointer); the result is copied to `host`, and `dev` of every buffer is
+ * cleared as the reference's halide_device_free does.  Unlike the
+ * reference's GPU scatter, every point is gridded (it drops the last
+ * n mod 200). */
+int kern_scatter_gpu1(const double scale, const int32_t grid_size,
+                      const int32_t margin_size, buffer_t *vis, buffer_t *gcf,
+                      buffer_t *uvg);
+int kern_degrid_gpu1(const double scale, const int32_t grid_size,
+                     const int32_t margin_size, buffer_t *gcf, buffer_t *uvg,
+                     buffer_t *vis, buffer_t *vis_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCGRID_HALIDE_H */

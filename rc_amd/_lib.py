@@ -115,6 +115,11 @@ SIGNATURES: dict[str, tuple] = {
     "mkGCFLayer": (vp, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
     "rcg_gcf_pad_quirk": (None, [C.c_int]),
     "calcAccums": (None, [vp, vp, vp, C.c_double, C.c_int, C.c_int]),
+    # MS6 Halide buffer_t entry points (buffer_t passed by pointer)
+    "kern_scatter": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp]),
+    "kern_degrid": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]),
+    "kern_scatter_gpu1": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp]),
+    "kern_degrid_gpu1": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]),
     # GCF generation (device)
     "rcg_gcf_gen_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_gcf_gen_layer": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

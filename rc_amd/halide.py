@@ -1,0 +1,75 @@
+"""ctypes view of the MS6 Halide buffer_t ABI (include/rcgrid_halide.h) and
+calls of kern_scatter / kern_degrid / kern_scatter_gpu1 / kern_degrid_gpu1.
+
+Array conventions (the MS6 generators' buffer layouts):
+  vis  float64 [n, 5]      (u, v, w, re, im)
+  gcf  float64 [64, S, S, 2]  [over][y][x][re|im], over = ou + 8*ov
+  uvg  float64 [Y, P, 2]   [y][x][re|im], P >= X columns; buffer min = (x0, y0)
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class BufferT(C.Structure):
+    _fields_ = [("dev", C.c_uint64), ("host", C.c_void_p), ("extent", C.c_int32 * 4),
+                ("stride", C.c_int32 * 4), ("min", C.c_int32 * 4), ("elem_size", C.c_int32),
+                ("host_dirty", C.c_bool), ("dev_dirty", C.c_bool),
+                ("_padding", C.c_uint8 * (10 - C.sizeof(C.c_void_p)))]
+
+
+assert C.sizeof(BufferT) == 72
+
+
+def _buf(arr: np.ndarray | None, extent, stride, mins=(0, 0, 0, 0), dev: int = 0,
+         host_dirty: bool = False) -> BufferT:
+    b = BufferT()
+    b.dev = dev
+    b.host = arr.ctypes.data if arr is not None else None
+    for i in range(4):
+        b.extent[i] = extent[i] if i < len(extent) else 0
+        b.stride[i] = stride[i] if i < len(stride) else 0
+        b.min[i] = mins[i] if i < len(mins) else 0
+    b.elem_size = 8
+    b.host_dirty = host_dirty
+    return b
+
+
+def vis_buffer(vis: np.ndarray, first: int = 0, dev: int = 0, host_dirty=False) -> BufferT:
+    assert vis.dtype == np.float64 and vis.ndim == 2 and vis.shape[1] == 5
+    assert vis.flags.c_contiguous
+    return _buf(vis, (5, vis.shape[0]), (1, 5), (0, first), dev, host_dirty)
+
+
+def gcf_buffer(gcf: np.ndarray, dev: int = 0, host_dirty=False) -> BufferT:
+    assert gcf.dtype == np.float64 and gcf.ndim == 4 and gcf.shape[0] == 64
+    assert gcf.shape[3] == 2 and gcf.shape[1] == gcf.shape[2] and gcf.flags.c_contiguous
+    S = gcf.shape[1]
+    return _buf(gcf, (2, S, S, 64), (1, 2, 2 * S, 2 * S * S), (0, 0, 0, 0), dev, host_dirty)
+
+
+def uvg_buffer(uvg: np.ndarray, width: int | None = None, origin=(0, 0), dev: int = 0,
+               host_dirty=False) -> BufferT:
+    assert uvg.dtype == np.float64 and uvg.ndim == 3 and uvg.shape[2] == 2
+    assert uvg.flags.c_contiguous
+    Y, P = uvg.shape[:2]
+    X = width or P
+    return _buf(uvg, (2, X, Y), (1, 2, 2 * P), (0, origin[0], origin[1]), dev, host_dirty)
+
+
+def scatter(scale, grid_size, margin, vis_b, gcf_b, uvg_b, gpu1: bool = False) -> int:
+    h = _lib.lib()
+    fn = h.kern_scatter_gpu1 if gpu1 else h.kern_scatter
+    return int(fn(scale, grid_size, margin, C.addressof(vis_b), C.addressof(gcf_b),
+                  C.addressof(uvg_b)))
+
+
+def degrid(scale, grid_size, margin, gcf_b, uvg_b, vis_b, out_b, gpu1: bool = False) -> int:
+    h = _lib.lib()
+    fn = h.kern_degrid_gpu1 if gpu1 else h.kern_degrid
+    return int(fn(scale, grid_size, margin, C.addressof(gcf_b), C.addressof(uvg_b),
+                  C.addressof(vis_b), C.addressof(out_b)))

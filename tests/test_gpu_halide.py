@@ -1,0 +1,95 @@
+"""GPU parity of the MS6 Halide buffer_t entry points (kern_scatter,
+kern_degrid and the _gpu1 variants) against oracle/halide_oracle.py.
+Bar: FP64, within 1e-12 of the largest value."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import halide_oracle as H
+from rc_amd import halide as HB
+
+pytestmark = pytest.mark.gpu
+
+
+def case(S, n=3000, seed=0, N=256):
+    rng = np.random.default_rng(seed + S)
+    lim = 100.0
+    vis = np.ascontiguousarray(np.column_stack([
+        rng.uniform(-lim, lim, n), rng.uniform(-lim, lim, n), rng.standard_normal(n),
+        rng.standard_normal(n), rng.standard_normal(n)]))
+    gcf = np.ascontiguousarray(rng.standard_normal((64, S, S, 2)))
+    return vis, gcf, N
+
+
+def close(a, b):
+    return np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("S", [8, 16, 32, 64])
+@pytest.mark.parametrize("gpu1", [False, True])
+def test_scatter_vs_oracle(hip_device, S, gpu1):
+    vis, gcf, N = case(S)
+    # a 200 x 180 window of the grid starting at (20, 30), row pitch 210
+    origin, width, height, pitch = (20, 30), 200, 180, 210
+    uvg = np.zeros((height, pitch, 2))
+    uvg[..., 0] = 0.5  # the kernel adds to what is there
+    exp = uvg.copy()
+    H.scatter(vis, gcf, exp, 0.9, N, 6, width=width, origin=origin)
+    rc = HB.scatter(0.9, N, 6, HB.vis_buffer(vis), HB.gcf_buffer(gcf),
+                    HB.uvg_buffer(uvg, width, origin), gpu1=gpu1)
+    assert rc == 0
+    assert close(uvg, exp)
+    assert np.count_nonzero(exp[..., 1]) > 0
+
+
+@pytest.mark.parametrize("S", [8, 32])
+@pytest.mark.parametrize("gpu1", [False, True])
+def test_degrid_vs_oracle(hip_device, S, gpu1):
+    vis, gcf, N = case(S, seed=5)
+    rng = np.random.default_rng(9)
+    origin, width, height, pitch = (10, 12), 230, 220, 236
+    uvg = np.ascontiguousarray(rng.standard_normal((height, pitch, 2)))
+    exp = H.degrid(gcf, uvg, vis, 0.9, N, 0, width=width, origin=origin)
+    out = np.zeros_like(vis)
+    rc = HB.degrid(0.9, N, 0, HB.gcf_buffer(gcf), HB.uvg_buffer(uvg, width, origin),
+                   HB.vis_buffer(vis), HB.vis_buffer(out), gpu1=gpu1)
+    assert rc == 0
+    assert close(out, exp)
+    assert np.array_equal(out[:, :3], vis[:, :3])
+    assert np.count_nonzero(out[:, 3] - vis[:, 3]) > 0
+
+
+def test_gpu1_reads_device_buffers_and_clears_dev(hip_device):
+    vis, gcf, N = case(16, seed=2)
+    dvis = torch.from_numpy(vis).cuda()
+    dgcf = torch.from_numpy(gcf).cuda()
+    uvg = np.zeros((N, N, 2))
+    exp = uvg.copy()
+    H.scatter(vis, gcf, exp, 0.9, N, 0)
+    # host copies poisoned (points far outside the grid, NaN taps): the
+    # device pointers must be used
+    vb = HB.vis_buffer(np.full_like(vis, 1e300), dev=dvis.data_ptr())
+    gb = HB.gcf_buffer(np.full_like(gcf, np.nan), dev=dgcf.data_ptr())
+    ub = HB.uvg_buffer(uvg)
+    assert HB.scatter(0.9, N, 0, vb, gb, ub, gpu1=True) == 0
+    assert close(uvg, exp)
+    assert vb.dev == 0 and gb.dev == 0 and ub.dev == 0
+    # a dirty host copy wins over the device pointer (zeros there)
+    zeros = torch.zeros_like(dvis)
+    uvg2 = np.zeros((N, N, 2))
+    vb2 = HB.vis_buffer(vis, dev=zeros.data_ptr(), host_dirty=True)
+    assert HB.scatter(0.9, N, 0, vb2, HB.gcf_buffer(gcf), HB.uvg_buffer(uvg2), gpu1=True) == 0
+    assert close(uvg2, exp)
+
+
+def test_unsupported_gcf_size_codes(hip_device):
+    vis, _, N = case(8, n=10)
+    gcf = np.zeros((64, 12, 12, 2))
+    uvg = np.zeros((N, N, 2))
+    out = np.zeros_like(vis)
+    args = (HB.vis_buffer(vis), HB.gcf_buffer(gcf), HB.uvg_buffer(uvg))
+    assert HB.scatter(1.0, N, 0, *args) == -444
+    assert HB.scatter(1.0, N, 0, *args, gpu1=True) == -222
+    dargs = (HB.gcf_buffer(gcf), HB.uvg_buffer(uvg), HB.vis_buffer(vis), HB.vis_buffer(out))
+    assert HB.degrid(1.0, N, 0, *dargs) == -555
+    assert HB.degrid(1.0, N, 0, *dargs, gpu1=True) == -333

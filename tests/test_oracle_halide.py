@@ -1,0 +1,103 @@
+"""Pinning of the MS6 Halide-kernel oracle (oracle/halide_oracle.py; the
+Halide generators cannot be built here): delta-tap KATs through the pixel
+and oversampling mapping, the bounds KAT, a scalar-loop brute force, and
+the bilinear transpose identity between scatter and degrid."""
+import numpy as np
+
+from oracle import halide_oracle as H
+
+
+def delta_gcf(S, over, y, x, val=1.0 + 0.0j):
+    g = np.zeros((64, S, S, 2))
+    g[over, y, x] = (val.real, val.imag)
+    return g
+
+
+def test_delta_tap_lands_on_its_cell():
+    S, N = 8, 64
+    for ou, ov, a, b in ((0, 0, 0, 0), (3, 5, 2, 7), (7, 1, 7, 0), (4, 6, 5, 3)):
+        # fractions (o + 0.25)/8 away from rounding halves; round() of the
+        # pixel goes up from o >= 4
+        u, v = 10 + (ou + 0.25) / 8, -7 + (ov + 0.25) / 8
+        vis = np.array([[u, v, 0.0, 2.0, -1.0]])
+        uvg = np.zeros((N, N, 2))
+        H.scatter(vis, delta_gcf(S, ou + 8 * ov, a, b), uvg, 1.0, N, 0)
+        x0 = 10 + (1 if ou >= 4 else 0) + N // 2 - S // 2
+        y0 = -7 + (1 if ov >= 4 else 0) + N // 2 - S // 2
+        nz = np.argwhere(uvg[..., 0] != 0)
+        assert nz.tolist() == [[y0 + a, x0 + b]]
+        assert tuple(uvg[y0 + a, x0 + b]) == (2.0, -1.0)
+
+
+def test_pixel_rounds_halves_to_even():
+    # Halide's round: 2.5 -> 2, 3.5 -> 4 (numpy.rint); oversampling offset
+    # 8 * 0.5 = 4 exactly stays 4
+    assert H.pixel(2.5, 3.5, 1.0, 16, 8)[:2] == (2 + 4, 4 + 4)
+    assert H.pixel(2.5, 3.5, 1.0, 16, 8)[2] == 4 + 8 * 4
+
+
+def test_bounds_are_inclusive_of_extent_minus_s_minus_one():
+    S, N = 8, 32
+    gcf = np.ones((64, S, S, 2))
+    for x0, inside in ((N - S - 1, True), (N - S, False), (0, True), (-1, False)):
+        u = x0 - (N // 2 - S // 2) + 0.1
+        vis = np.array([[u, 0.1, 0, 1.0, 0.0]])
+        uvg = np.zeros((N, N, 2))
+        H.scatter(vis, gcf, uvg, 1.0, N, 0)
+        assert bool(uvg.any()) == inside, x0
+    # a margin of 12 with S = 8 shrinks the range by 2 on each side
+    assert H.in_bounds(2, 2, 8, 32, 32, (0, 0), 12)
+    assert not H.in_bounds(1, 2, 8, 32, 32, (0, 0), 12)
+    assert not H.in_bounds(N - S - 2, 2, 8, 32, 32, (0, 0), 12)
+
+
+def brute_scatter(vis, gcf, uvg, scale, N, margin, origin):
+    S = gcf.shape[1]
+    Y, X = uvg.shape[:2]
+    for u, v, _, re, im in vis:
+        us, vs = u * scale, v * scale
+        ou = min(max(int(np.rint(8 * (us - np.floor(us)))), 0), 7)
+        ov = min(max(int(np.rint(8 * (vs - np.floor(vs)))), 0), 7)
+        x0 = int(np.rint(us)) + N // 2 - S // 2
+        y0 = int(np.rint(vs)) + N // 2 - S // 2
+        m = max(0, (margin - S) // 2)
+        if not (origin[0] + m <= x0 <= origin[0] + X - S - 1 - m and
+                origin[1] + m <= y0 <= origin[1] + Y - S - 1 - m):
+            continue
+        for sy in range(S):
+            for sx in range(S):
+                gr, gi = gcf[ou + 8 * ov, sy, sx]
+                uvg[y0 + sy - origin[1], x0 + sx - origin[0], 0] += re * gr - im * gi
+                uvg[y0 + sy - origin[1], x0 + sx - origin[0], 1] += re * gi + im * gr
+
+
+def test_scatter_matches_scalar_loops():
+    rng = np.random.default_rng(0)
+    S, N = 8, 48
+    vis = np.column_stack([rng.uniform(-30, 30, 60), rng.uniform(-30, 30, 60),
+                           rng.standard_normal(60), rng.standard_normal(60),
+                           rng.standard_normal(60)])
+    gcf = rng.standard_normal((64, S, S, 2))
+    a = np.zeros((40, 44, 2))
+    b = np.zeros((40, 44, 2))
+    H.scatter(vis, gcf, a, 0.7, N, 10, origin=(3, 5))
+    brute_scatter(vis, gcf, b, 0.7, N, 10, origin=(3, 5))
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
+    assert np.count_nonzero(a) > 0
+
+
+def test_degrid_is_the_transpose_of_scatter():
+    rng = np.random.default_rng(1)
+    S, N = 16, 64
+    vis = np.column_stack([rng.uniform(-20, 20, 40), rng.uniform(-20, 20, 40),
+                           np.zeros(40), rng.standard_normal(40), rng.standard_normal(40)])
+    gcf = rng.standard_normal((64, S, S, 2))
+    g = rng.standard_normal((N, N, 2))
+    sc = np.zeros((N, N, 2))
+    H.scatter(vis, gcf, sc, 1.0, N, 0)
+    lhs = np.sum((sc[..., 0] + 1j * sc[..., 1]) * (g[..., 0] + 1j * g[..., 1]))
+    out = H.degrid(gcf, g, vis, 1.0, N, 0)
+    dg = (vis[:, 3] - out[:, 3]) + 1j * (vis[:, 4] - out[:, 4])
+    rhs = np.sum((vis[:, 3] + 1j * vis[:, 4]) * dg)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    assert np.array_equal(out[:, :3], vis[:, :3])
