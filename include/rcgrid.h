@@ -319,6 +319,16 @@ rcg_status rcg_subtract_vis(void *out, const void *in, int f64, long long n,
                             void *stream);
 rcg_status rcg_const_vis(void *vis, int f64, long long n, double re, double im,
                          void *stream);
+/* Ingest of the MS6 visibility layout, five doubles (u, v, w, re, im) per
+ * point (MS6/programs/Kernel/IO.hs:59-71): split into uvw and complex
+ * visibilities (complex128 if f64, else complex64), with rotate != 0
+ * multiplying each visibility by exp(2 pi i w scale) in FP64 (rotw,
+ * MS4/kernel/common/common.h:90-94, as MS4/kernel/cpu/binner/binner.cpp:135
+ * does while binning); and the way back. */
+rcg_status rcg_unpack_vis5(const double *rec, long long n, int rotate, double scale,
+                           rcg_double3 *uvw, void *vis, int f64, void *stream);
+rcg_status rcg_pack_vis5(const rcg_double3 *uvw, const void *vis, int f64,
+                         long long n, double *rec, void *stream);
 rcg_status rcg_fftshift_even(void *data, int elem_bytes, int size, int pitch,
                              void *stream);
 

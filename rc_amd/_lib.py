@@ -94,6 +94,8 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_herm": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "rcg_subtract_vis": (C.c_int, [vp, vp, C.c_int, C.c_longlong, vp]),
     "rcg_const_vis": (C.c_int, [vp, C.c_int, C.c_longlong, C.c_double, C.c_double, vp]),
+    "rcg_unpack_vis5": (C.c_int, [vp, C.c_longlong, C.c_int, C.c_double, vp, vp, C.c_int, vp]),
+    "rcg_pack_vis5": (C.c_int, [vp, vp, C.c_int, C.c_longlong, vp, vp]),
     "rcg_fftshift_even": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "rcg_fft_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_fft_execute": (C.c_int, [vp, vp, vp]),

@@ -133,6 +133,54 @@ __global__ void k_subtract(C *__restrict__ out, const C *__restrict__ in, long l
   }
 }
 
+// The MS6 visibility layout, five doubles (u, v, w, re, im) per point
+// (MS6/programs/Kernel/IO.hs:59-71), split into uvw and visibilities, the
+// visibility optionally multiplied by exp(2 pi i w scale) in FP64 (rotw,
+// MS4/kernel/common/common.h:90-94, as the binner applies it,
+// MS4/kernel/cpu/binner/binner.cpp:135).
+template <class C>
+__global__ void k_unpack5(const double *__restrict__ rec, long long n, int rotate, double scale,
+                          rcg_double3 *__restrict__ uvw, C *__restrict__ vis)
+{
+#pragma clang fp contract(off)
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double *r = rec + 5 * i;
+    const double u = r[0], v = r[1], w = r[2];
+    double re = r[3], im = r[4];
+    if (rotate) {
+      double s, c;
+      sincos(2.0 * M_PI * (w * scale), &s, &c);
+      const double xr = re * c - im * s, xi = re * s + im * c;
+      re = xr;
+      im = xi;
+    }
+    uvw[i].u = u;
+    uvw[i].v = v;
+    uvw[i].w = w;
+    C o;
+    o.x = re;
+    o.y = im;
+    vis[i] = o;
+  }
+}
+
+// Back to five doubles per point.
+template <class C>
+__global__ void k_pack5(const rcg_double3 *__restrict__ uvw, const C *__restrict__ vis,
+                        long long n, double *__restrict__ rec)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double *r = rec + 5 * i;
+    r[0] = uvw[i].u;
+    r[1] = uvw[i].v;
+    r[2] = uvw[i].w;
+    r[3] = (double)vis[i].x;
+    r[4] = (double)vis[i].y;
+  }
+}
+
 // vis = value (constVis, MS4/dna-programs/Data.hs:242-247: the PSF's
 // visibilities are all 1).
 template <class C, class E>
@@ -351,6 +399,46 @@ extern "C" rcg_status rcg_subtract_vis(void *out, const void *in, int f64, long 
   else
     hipLaunchKernelGGL(k_subtract<float2>, dim3(nb), dim3(256), 0, (hipStream_t)stream,
                        static_cast<float2 *>(out), static_cast<const float2 *>(in), n);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_unpack_vis5(const double *rec, long long n, int rotate, double scale,
+                                      rcg_double3 *uvw, void *vis, int f64, void *stream)
+{
+  clear_error();
+  if (n < 0 || (n > 0 && (!rec || !uvw || !vis))) {
+    set_error(RCG_ERR_ARG, "rcg_unpack_vis5: bad arguments");
+    return RCG_ERR_ARG;
+  }
+  if (n == 0) return RCG_OK;
+  const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+  if (f64)
+    hipLaunchKernelGGL(k_unpack5<double2>, dim3(nb), dim3(256), 0, (hipStream_t)stream, rec, n,
+                       rotate, scale, uvw, static_cast<double2 *>(vis));
+  else
+    hipLaunchKernelGGL(k_unpack5<float2>, dim3(nb), dim3(256), 0, (hipStream_t)stream, rec, n,
+                       rotate, scale, uvw, static_cast<float2 *>(vis));
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_pack_vis5(const rcg_double3 *uvw, const void *vis, int f64,
+                                    long long n, double *rec, void *stream)
+{
+  clear_error();
+  if (n < 0 || (n > 0 && (!rec || !uvw || !vis))) {
+    set_error(RCG_ERR_ARG, "rcg_pack_vis5: bad arguments");
+    return RCG_ERR_ARG;
+  }
+  if (n == 0) return RCG_OK;
+  const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+  if (f64)
+    hipLaunchKernelGGL(k_pack5<double2>, dim3(nb), dim3(256), 0, (hipStream_t)stream, uvw,
+                       static_cast<const double2 *>(vis), n, rec);
+  else
+    hipLaunchKernelGGL(k_pack5<float2>, dim3(nb), dim3(256), 0, (hipStream_t)stream, uvw,
+                       static_cast<const float2 *>(vis), n, rec);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }

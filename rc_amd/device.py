@@ -356,3 +356,33 @@ def hogbom(res: torch.Tensor, psf: torch.Tensor, niters: int, gain: float, thres
     _lib.check(h.rcg_hogbom_iterations(scratch.data_ptr(), C.byref(it), s),
                "rcg_hogbom_iterations")
     return model, int(it.value)
+
+
+def unpack_vis5(rec: torch.Tensor, f64: bool = False, rotate_scale: float | None = None,
+                stream=None) -> tuple[torch.Tensor, torch.Tensor]:
+    """MS6 five-double records [n, 5] (u, v, w, re, im) on the device ->
+    (uvw float64 [n, 3], visibilities complex64/complex128 [n]), optionally
+    rotated by exp(2 pi i w scale)."""
+    if rec.dtype != torch.float64 or rec.dim() != 2 or rec.shape[1] != 5 or \
+            not rec.is_contiguous():
+        raise ValueError("records must be a contiguous float64 [n, 5] tensor")
+    n = rec.shape[0]
+    uvw = torch.empty((n, 3), dtype=torch.float64, device=rec.device)
+    vis = torch.empty(n, dtype=torch.complex128 if f64 else torch.complex64, device=rec.device)
+    rot = rotate_scale is not None
+    _lib.check(_lib.lib().rcg_unpack_vis5(rec.data_ptr(), n, int(rot),
+                                          float(rotate_scale or 0.0), uvw.data_ptr(),
+                                          vis.data_ptr(), int(f64), _stream(stream)),
+               "rcg_unpack_vis5")
+    return uvw, vis
+
+
+def pack_vis5(uvw: torch.Tensor, vis: torch.Tensor, stream=None) -> torch.Tensor:
+    """(uvw [n, 3], vis [n]) -> MS6 records [n, 5] float64."""
+    n = vis.numel()
+    f64 = vis.dtype == torch.complex128
+    rec = torch.empty((n, 5), dtype=torch.float64, device=vis.device)
+    _lib.check(_lib.lib().rcg_pack_vis5(uvw.contiguous().data_ptr(), vis.contiguous().data_ptr(),
+                                        int(f64), n, rec.data_ptr(), _stream(stream)),
+               "rcg_pack_vis5")
+    return rec

@@ -141,3 +141,25 @@ def test_hogbom_device_fp32_iterations_and_threshold(hip_device):
     assert it.value == it_ref < 1000
     assert rel_l2(mod.cpu().numpy(), mod_ref) <= 1e-5
     assert np.abs(res.cpu().numpy()).max() < 1e-2
+
+
+def test_vis5_unpack_rotate_and_pack(hip_device):
+    """MS6 five-double records -> uvw + visibilities (rotated by
+    exp(2 pi i w scale) in FP64 against the oracle's rotateCPU) and back."""
+    from rc_amd import device as D
+    rng = np.random.default_rng(12)
+    n = 5000
+    rec = rng.standard_normal((n, 5)) * np.array([100, 100, 30, 1, 1])
+    t = torch.from_numpy(rec).cuda()
+    uvw, vis = D.unpack_vis5(t, f64=True)
+    assert np.array_equal(uvw.cpu().numpy(), rec[:, :3])
+    assert np.array_equal(vis.cpu().numpy(), rec[:, 3] + 1j * rec[:, 4])
+    uvw, vrot = D.unpack_vis5(t, f64=True, rotate_scale=0.013)
+    exp = (rec[:, 3] + 1j * rec[:, 4]).copy()
+    O.oracle_lib().orc_rotate(np.ascontiguousarray(rec[:, :3]).ctypes.data, exp.ctypes.data, n,
+                              0.013)
+    assert rel_l2(vrot.cpu().numpy(), exp) <= 1e-15
+    back = D.pack_vis5(uvw, vis).cpu().numpy()
+    assert np.array_equal(back, rec)
+    _, v32 = D.unpack_vis5(t, f64=False)
+    assert np.array_equal(v32.cpu().numpy(), (rec[:, 3] + 1j * rec[:, 4]).astype(np.complex64))
