@@ -156,8 +156,8 @@ def test_vis5_unpack_rotate_and_pack(hip_device):
     assert np.array_equal(vis.cpu().numpy(), rec[:, 3] + 1j * rec[:, 4])
     uvw, vrot = D.unpack_vis5(t, f64=True, rotate_scale=0.013)
     exp = (rec[:, 3] + 1j * rec[:, 4]).copy()
-    O.oracle_lib().orc_rotate(np.ascontiguousarray(rec[:, :3]).ctypes.data, exp.ctypes.data, n,
-                              0.013)
+    uvw_np = np.ascontiguousarray(rec[:, :3])  # kept alive across the call
+    O.oracle_lib().orc_rotate(uvw_np.ctypes.data, exp.ctypes.data, n, 0.013)
     assert rel_l2(vrot.cpu().numpy(), exp) <= 1e-15
     back = D.pack_vis5(uvw, vis).cpu().numpy()
     assert np.array_equal(back, rec)
