@@ -378,19 +378,6 @@ __device__ __forceinline__ void for_each_item(const TiledDev &d, int *__restrict
 
 // ---------------------------------------------------------------------------
 // Walking a pair's window/tile intersection.
-//
-// A wavefront loads the records (and, for gridding, the visibilities) of up
-// to 64 pairs at once, one per lane, and then walks them in order, the pair
-// fields broadcast from their lane with readlane (uniform, scalar).  For a
-// pair whose intersection is h x w cells, lane l takes elements
-// e = l, l + 64, l + 128, ... of the row-major flattening; each step moves
-// its (row, col) by (64 / w, 64 % w) with at most one wrap, and its tap and
-// LDS offsets by precomputed increments, so the loop has no multiplies and
-// no branches: a lane past the end loads its tap through a buffer
-// descriptor at an offset beyond the table (which reads 0) and points its
-// LDS access at a pad cell.  JOB_U steps are issued before they are
-// consumed, so JOB_U tap loads per lane are in flight.
-
 
 constexpr unsigned TAP_OOB = 0x7ffffff0u;  // byte offset past any tap table
 
@@ -420,31 +407,35 @@ __device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned vo
 // l / w, column l % w; lanes past rows*w idle); slab s covers intersection
 // rows [s*rows, (s+1)*rows), the last one possibly partial.  A lane's tap
 // offset for slab s is its slab-0 offset plus the uniform s*rows*stride (the
-// buffer load's scalar offset); its LDS index advances by rows*LS per slab.
-// A lane outside the intersection in a slab loads its tap at an
+// buffer load's scalar offset); its LDS byte address advances by rows*LS*8
+// per slab.  A lane outside the intersection in a slab loads its tap at an
 // out-of-range offset (which reads zero) and points its LDS access at the
-// copy's dummy cell.
+// copy's dummy cell.  The pair fields come from readlane and stay in scalar
+// registers: nothing below selects between two of them under a vector mask
+// (a VALU instruction reads at most one scalar operand), which would move
+// them, and every loop-carried copy of them, into vector registers.
 //
 // Latency: a wavefront loads the records (and visibilities) of 64 pairs at
 // once, one per lane, double-buffered one batch ahead, and broadcasts them
 // with readlane; the taps of a pair's first group of up to GRP slabs are
-// loaded while the previous pair's LDS updates run.  Three slabs per group
+// loaded while the previous pair's LDS updates run.  Two slabs per group
 // balance the loads kept in flight against the slots a short pair leaves
-// empty (measured on the 4096^2 w-projection workload: 2 and 4 slabs are
-// 4-10% slower, 1 and 6 slabs 17-20% slower).
-constexpr int GRP = 3;
+// empty (measured on the 4096^2 w-projection workload, T = 32: 3 slabs are
+// 3-6% slower, 4 slabs 10-15%, 1 slab 13-17%).
+constexpr int GRP = 2;
 
 struct Pair {
   int h, rows, nslabs, point, whole, conj, slot;
   int lr;          // lane row (idle lanes: 64)
   unsigned tb0;    // tap byte offset of the lane's slab-0 cell
-  int li0;         // LDS index of the lane's slab-0 cell
+  int lb0;         // LDS byte address of the lane's slab-0 cell
   int tstep;       // tap byte step per slab (uniform)
-  int lstep;       // LDS index step per slab
+  int lstep;       // LDS byte step per slab (uniform)
 };
 
-template <int LS, int DUMMY>
-__device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int lane)
+// base: LDS byte address of the tile copy (row-major, row stride LS)
+template <int LS>
+__device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int lane, int base)
 {
   Pair p;
   const int stride = pz & 0xffff;
@@ -467,41 +458,35 @@ __device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int 
   const bool idle = lr >= p.rows;
   p.lr = idle ? 64 : lr;
   p.tb0 = idle ? TAP_OOB : (unsigned)(py + __mul24(lr, stride) + lc) * 8u;
-  p.li0 = idle ? DUMMY : (r0 + lr) * LS + c0 + lc;
+  p.lb0 = base + ((r0 + lr) * LS + c0 + lc) * 8;
   p.tstep = p.rows * stride * 8;
-  p.lstep = p.rows * LS;
+  p.lstep = p.rows * LS * 8;
   return p;
 }
 
-// taps of slabs [s0, s0 + GRP) of pair p (zero outside the intersection)
-__device__ __forceinline__ void load_group(float2 (&t)[GRP], const Pair &p, int s0,
-                                           __amdgpu_buffer_rsrc_t rsrc)
+// Taps and LDS byte addresses of slabs [s0, s0 + GRP) of pair p: outside
+// the intersection the tap reads zero and the address is `dummy`.  One
+// compare per slab serves both selects.
+__device__ __forceinline__ void load_group(float2 (&t)[GRP], int (&la)[GRP], const Pair &p,
+                                           int s0, int dummy, __amdgpu_buffer_rsrc_t rsrc)
 {
 #pragma unroll
   for (int u = 0; u < GRP; u++) {
-    const int rs = (s0 + u) * p.rows;
-    const bool act = p.lr < p.h - rs;
+    const bool act = p.lr < p.h - (s0 + u) * p.rows;
     t[u] = tap_load(rsrc, act ? p.tb0 : TAP_OOB, (s0 + u) * p.tstep);
+    la[u] = act ? p.lb0 + (s0 + u) * p.lstep : dummy;
   }
 }
 
-__device__ __forceinline__ int group_lds(const Pair &p, int s, int dummy)
+__device__ __forceinline__ float2 &lds_at(float2 *lds, int byte)
 {
-  const int rs = s * p.rows;
-  return p.lr < p.h - rs ? p.li0 + s * p.lstep : dummy;
+  return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + byte);
 }
 
-// Stream of work for one wavefront: its pairs ir.begin + wave + 4k, each cut
-// into groups of GRP slabs.  Records (and visibilities) come 64 at a time,
-// one per lane, double-buffered a batch ahead and broadcast with readlane.
-struct Stream {
-  int npw;       // pairs of this wavefront
-  int k;         // current pair
-  int s0;        // first slab of the current group
-  int4 rec, rec_n;
-  float2 mv, mv_n;
-};
-
+// A wavefront's pairs are ir.begin + wave + 4k, each cut into groups of GRP
+// slabs.  Their records (and visibilities) come 64 at a time, one per lane,
+// double-buffered a batch ahead and broadcast with readlane; batch b is
+// loaded by stream_batch.
 template <bool VIS>
 __device__ __forceinline__ void stream_batch(const TiledDev &d, const float2 *__restrict__ vis,
                                              const ItemRange &ir, int wave, int lane, int npw,
@@ -547,7 +532,8 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  float2 *my_acc = acc + wave * CP;
+  // LDS byte addresses of the wavefront's copy and of its dummy cell
+  const int base = wave * CP * 8, dummy = (wave * CP + DUMMY) * 8;
   for_each_item(d, d.queue, [&](const ItemRange &ir) {
     for (int q = threadIdx.x; q < 4 * CP; q += 256) acc[q] = make_float2(0.f, 0.f);
     __syncthreads();
@@ -558,18 +544,18 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv);
       stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n);
       int k = 0, s0 = 0;
-      Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0),
-                                      lane);
+      Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane,
+                               base);
       float vx = rlf(mv.x, 0), vy = rlf(mv.y, 0);
       float2 ta[GRP], tb[GRP];
-      load_group(ta, p, 0, rsrc);
+      int la[GRP], lb[GRP];
+      load_group(ta, la, p, 0, dummy, rsrc);
       bool more = true;
       // One group: choose the next group (the same pair's next slabs, or the
-      // next pair's first), issue its taps into tn (past the end pn == p: a
-      // harmless reload), then update the current group's cells from tc
-      // while they fly.  Called alternately with the two buffers, so no
+      // next pair's first; past the end, the empty pair), issue its taps into
+      // tn, then update the current group's cells from tc while they fly.  Called alternately with the two buffers, so no
       // register copy forces a wait on the loads in flight.
-      auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
+      auto body = [&](float2 (&tc)[GRP], int (&lc)[GRP], float2 (&tn)[GRP], int (&ln)[GRP]) {
         Pair pn = p;
         float vxn = vx, vyn = vy;
         int kn = k, sn = s0 + GRP;
@@ -583,45 +569,49 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
               mv = mv_n;
               stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n);
             }
-            pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j),
-                                        rl(rec.w, j), lane);
+            pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
+                                 base);
             vxn = rlf(mv.x, j);
             vyn = rlf(mv.y, j);
+          } else {
+            pn.h = 0;  // past the end: the empty pair, no lane active
           }
         }
         more = kn < npw;
-        load_group(tn, pn, sn, rsrc);
+        load_group(tn, ln, pn, sn, dummy, rsrc);
         __builtin_amdgcn_sched_barrier(0);
         // taps are conjugated for mirrored (w < 0) layers and, in the
-        // adjoint, once more for all of them
-        const bool cj = p.conj != d.adj;
-        const float a = vx, b = cj ? vy : -vy, c = cj ? -vx : vx, e = vy;
-        int li[GRP];
+        // adjoint, once more for all of them: b = cj ? vy : -vy and
+        // c = cj ? -vx : vx, as sign-bit flips (scalar integer ops)
+        const unsigned flip = p.conj != d.adj ? 0u : 0x80000000u;
+        const float a = vx, e = vy;
+        const float b = __int_as_float(__float_as_int(vy) ^ flip);
+        const float c = __int_as_float(__float_as_int(vx) ^ flip ^ 0x80000000u);
         float2 x[GRP];
-#pragma unroll
-        for (int u = 0; u < GRP; u++) li[u] = group_lds(p, s0 + u, DUMMY);
         // cells of one pair are distinct, so the reads may precede the writes
 #pragma unroll
-        for (int u = 0; u < GRP; u++) x[u] = my_acc[li[u]];
+        for (int u = 0; u < GRP; u++) x[u] = lds_at(acc, lc[u]);
 #pragma unroll
         for (int u = 0; u < GRP; u++) {
           x[u].x = fmaf(a, tc[u].x, fmaf(b, tc[u].y, x[u].x));
           x[u].y = fmaf(c, tc[u].y, fmaf(e, tc[u].x, x[u].y));
         }
 #pragma unroll
-        for (int u = 0; u < GRP; u++) my_acc[li[u]] = x[u];
+        for (int u = 0; u < GRP; u++) lds_at(acc, lc[u]) = x[u];
         p = pn;
         vx = vxn;
         vy = vyn;
         k = kn;
         s0 = sn;
       };
-      while (true) {
-        body(ta, tb);
-        if (!more) break;
-        body(tb, ta);
-        if (!more) break;
-      }
+      // Both bodies run on every trip (the second one past the end works on
+      // the empty pair and touches only the dummy cell): a loop exit between
+      // them would join the back edge with undefined state, which costs
+      // vector copies on every group.
+      do {
+        body(ta, la, tb, lb);
+        body(tb, lb, ta, la);
+      } while (more);
       // Never true (chunk >= 1): a use of both buffers on the loop exits
       // keeps the compiler from sinking the prefetch loads past the exit
       // tests.
@@ -632,7 +622,7 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
           z.x += ta[u].x + tb[u].x;
           z.y += ta[u].y + tb[u].y;
         }
-        my_acc[lane] = z;
+        acc[wave * CP + lane] = z;
       }
     }
     __syncthreads();
@@ -687,12 +677,24 @@ __global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restri
 
 // ---------------------------------------------------------------------------
 // degridding: the grid tile is staged once in LDS (read-only); per pair the
-// lanes accumulate four partial products, then one wavefront reduction.
+// lanes accumulate four partial products, which are summed across the
+// wavefront four pairs at a time.
 
-// Sum over the 64 lanes with DPP: prefix sums within each row of 16 lanes
-// (row_shr 1, 2, 4, 8 with zero fill), then the four row totals (lanes 15,
-// 31, 47, 63) are read into scalars.
-__device__ __forceinline__ float wave_sum(float x)
+// Sums of 4 pairs at once.  When a pair ends, every lane leaves its part of
+// the pair's sum in the wavefront's LDS buffer, row slot*64 + lane, and lane
+// 0 the output address in meta[slot].  After 4 pairs (or at the end of an
+// item) lane l sums entries c, c + 16, c + 32, c + 48 of pair l / 16 (c =
+// l % 16), then a DPP prefix sum over its row of 16 lanes (row_shr 1, 2,
+// 4, 8 with zero fill) leaves the pair's total in the row's last lane,
+// which stores it.  The order of the additions is fixed, so the result is
+// reproducible.
+constexpr int SUMS = 4;
+// Output addresses kept in LDS are typed global, so the final store is a
+// global (not flat) store and does not hold up the wavefront's LDS waits;
+// the (re, im) pair is stored as one 64-bit word, re in the low half.
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+
+__device__ __forceinline__ float row_sum(float x)
 {
   int v = __float_as_int(x);
   x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
@@ -702,12 +704,26 @@ __device__ __forceinline__ float wave_sum(float x)
   x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
   v = __float_as_int(x);
   x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
-  v = __float_as_int(x);
-  const float a = __int_as_float(__builtin_amdgcn_readlane(v, 15));
-  const float b = __int_as_float(__builtin_amdgcn_readlane(v, 31));
-  const float c = __int_as_float(__builtin_amdgcn_readlane(v, 47));
-  const float e = __int_as_float(__builtin_amdgcn_readlane(v, 63));
-  return (a + b) + (c + e);
+  return x;
+}
+
+// n: pairs in the buffer (1..SUMS); entries of the other rows are stale
+// and their sums are not stored
+__device__ __forceinline__ void flush_sums(const float2 *buf, gu64 *const *meta, int n,
+                                           int lane)
+{
+  // the entries come from other lanes of the wavefront: keep the compiler
+  // from moving LDS accesses across this point (the hardware runs a
+  // wavefront's LDS instructions in order)
+  __builtin_amdgcn_wave_barrier();
+  const int q = lane >> 4, c = lane & 15;
+  const float2 *b = buf + q * 64 + c;
+  const float2 a0 = b[0], a1 = b[16], a2 = b[32], a3 = b[48];
+  const float re = row_sum((a0.x + a1.x) + (a2.x + a3.x));
+  const float im = row_sum((a0.y + a1.y) + (a2.y + a3.y));
+  if (c == 15 && q < n)
+    *meta[q] = (unsigned long long)__float_as_uint(im) << 32 | __float_as_uint(re);
+  __builtin_amdgcn_wave_barrier();
 }
 
 template <int T>
@@ -719,9 +735,13 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
   constexpr int LS = T + 1;
   constexpr int DUMMY = T * LS;
   __shared__ float2 tile[T * LS + 2];
+  __shared__ float2 sums[4][SUMS * 64];
+  __shared__ gu64 *meta[4][SUMS];
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float2 *my_sums = sums[wave];
+  gu64 **my_meta = meta[wave];
   for_each_item(d, d.queue + 8, [&](const ItemRange &ir) {
     const int gsz = d.grid_size;
     const float2 *gi = grid + ir.goff;
@@ -740,15 +760,16 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm);
       stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n);
       int k = 0, s0 = 0;
-      Pair p = decode_pair<LS, DUMMY>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0),
-                                      lane);
+      Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
       float2 ta[GRP], tb[GRP];
-      load_group(ta, p, 0, rsrc);
+      int la[GRP], lb[GRP];
+      load_group(ta, la, p, 0, DUMMY * 8, rsrc);
       // P = sum gx*tx, Q = sum gy*ty, R = sum gx*ty, U = sum gy*tx
       float P = 0.f, Q = 0.f, R = 0.f, U = 0.f;
+      int ns = 0;  // pairs in the sum buffer
       bool more = true;
       // one group; alternate buffers as in the gridder
-      auto body = [&](float2 (&tc)[GRP], float2 (&tn)[GRP]) {
+      auto body = [&](float2 (&tc)[GRP], int (&lc)[GRP], float2 (&tn)[GRP], int (&ln)[GRP]) {
         Pair pn = p;
         int kn = k, sn = s0 + GRP;
         const bool last = sn >= p.nslabs;
@@ -761,16 +782,18 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
               rec = rec_n;
               stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n);
             }
-            pn = decode_pair<LS, DUMMY>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j),
-                                        rl(rec.w, j), lane);
+            pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
+                                 0);
+          } else {
+            pn.h = 0;  // past the end: the empty pair, no lane active
           }
         }
         more = kn < npw;
-        load_group(tn, pn, sn, rsrc);
+        load_group(tn, ln, pn, sn, DUMMY * 8, rsrc);
         __builtin_amdgcn_sched_barrier(0);
         float2 g[GRP];
 #pragma unroll
-        for (int u = 0; u < GRP; u++) g[u] = tile[group_lds(p, s0 + u, DUMMY)];
+        for (int u = 0; u < GRP; u++) g[u] = lds_at(tile, lc[u]);
 #pragma unroll
         for (int u = 0; u < GRP; u++) {
           P = fmaf(g[u].x, tc[u].x, P);
@@ -778,18 +801,19 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
           R = fmaf(g[u].x, tc[u].y, R);
           U = fmaf(g[u].y, tc[u].x, U);
         }
-        if (last) {
-          // tap t' = (tx, sg*ty): re = P - sg*Q, im = sg*R + U
-          float re = p.conj ? P + Q : P - Q;
-          float im = p.conj ? U - R : U + R;
-          re = wave_sum(re);
-          im = wave_sum(im);
+        if (last && k < npw) {
+          // tap t' = (tx, sg*ty): re = P - sg*Q, im = sg*R + U (sg = +-1, so
+          // each fma rounds as the add or subtract would)
+          const float sg = p.conj ? -1.f : 1.f;
+          my_sums[ns * 64 + lane] = make_float2(fmaf(-sg, Q, P), fmaf(sg, R, U));
           // a point inside one tile is final; a split point leaves its part
           // in its slot for k_degrid_reduce
-          if (lane == 0) {
-            float2 *o = p.whole ? vis + p.point
-                                : d.partial + (long long)p.point * d.mpp + p.slot;
-            *o = make_float2(re, im);
+          if (lane == 0)
+            my_meta[ns] = (gu64 *)(p.whole ? vis + p.point
+                                           : d.partial + (long long)p.point * d.mpp + p.slot);
+          if (++ns == SUMS) {
+            flush_sums(my_sums, my_meta, SUMS, lane);
+            ns = 0;
           }
           P = Q = R = U = 0.f;
         }
@@ -797,12 +821,15 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
         k = kn;
         s0 = sn;
       };
-      while (true) {
-        body(ta, tb);
-        if (!more) break;
-        body(tb, ta);
-        if (!more) break;
-      }
+      // Both bodies run on every trip (the second one past the end works on
+      // the empty pair and touches only the dummy cell): a loop exit between
+      // them would join the back edge with undefined state, which costs
+      // vector copies on every group.
+      do {
+        body(ta, la, tb, lb);
+        body(tb, lb, ta, la);
+      } while (more);
+      if (ns > 0) flush_sums(my_sums, my_meta, ns, lane);
       // never true (chunk >= 1): keeps the prefetch loads from sinking
       if (d.chunk < 0) {
         float2 z = make_float2(0.f, 0.f);
