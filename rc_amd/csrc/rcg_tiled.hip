@@ -556,6 +556,14 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       // tn, then update the current group's cells from tc while they fly.  Called alternately with the two buffers, so no
       // register copy forces a wait on the loads in flight.
       auto body = [&](float2 (&tc)[GRP], int (&lc)[GRP], float2 (&tn)[GRP], int (&ln)[GRP]) {
+        // The current group's cells, read first so that the LDS latency
+        // overlaps the choice of the next group.  A wavefront's LDS
+        // operations complete in order, so the reads see the previous
+        // group's writes; the cells of one group are distinct, so all its
+        // reads may precede its writes.
+        float2 x[GRP];
+#pragma unroll
+        for (int u = 0; u < GRP; u++) x[u] = lds_at(acc, lc[u]);
         Pair pn = p;
         float vxn = vx, vyn = vy;
         int kn = k, sn = s0 + GRP;
@@ -587,10 +595,6 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
         const float a = vx, e = vy;
         const float b = __int_as_float(__float_as_int(vy) ^ flip);
         const float c = __int_as_float(__float_as_int(vx) ^ flip ^ 0x80000000u);
-        float2 x[GRP];
-        // cells of one pair are distinct, so the reads may precede the writes
-#pragma unroll
-        for (int u = 0; u < GRP; u++) x[u] = lds_at(acc, lc[u]);
 #pragma unroll
         for (int u = 0; u < GRP; u++) {
           x[u].x = fmaf(a, tc[u].x, fmaf(b, tc[u].y, x[u].x));
@@ -770,6 +774,11 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       bool more = true;
       // one group; alternate buffers as in the gridder
       auto body = [&](float2 (&tc)[GRP], int (&lc)[GRP], float2 (&tn)[GRP], int (&ln)[GRP]) {
+        // the current group's grid cells, read first so that the LDS
+        // latency overlaps the choice of the next group
+        float2 g[GRP];
+#pragma unroll
+        for (int u = 0; u < GRP; u++) g[u] = lds_at(tile, lc[u]);
         Pair pn = p;
         int kn = k, sn = s0 + GRP;
         const bool last = sn >= p.nslabs;
@@ -791,9 +800,6 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
         more = kn < npw;
         load_group(tn, ln, pn, sn, DUMMY * 8, rsrc);
         __builtin_amdgcn_sched_barrier(0);
-        float2 g[GRP];
-#pragma unroll
-        for (int u = 0; u < GRP; u++) g[u] = lds_at(tile, lc[u]);
 #pragma unroll
         for (int u = 0; u < GRP; u++) {
           P = fmaf(g[u].x, tc[u].x, P);
