@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tile", type=int, default=32)
-    ap.add_argument("--chunk", type=int, default=512)
+    ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--n-times", type=int, default=306)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -519,9 +519,9 @@ def main() -> int:
             "valu_tflops": round(8 * taps / (t_dom / 1e3) / 1e12, 3),
             "valu_frac": round(8 * taps / (t_dom / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)}
 
-    # measured HBM bytes of this kernel (profiled at tile 32, chunk 512 on the
+    # measured HBM bytes of this kernel (profiled at tile 32, chunk 1024 on the
     # default workload only)
-    if args.tile == 32 and args.chunk == 512 and args.n_times == 306:
+    if args.tile == 32 and args.chunk == 1024 and args.n_times == 306:
         tb = profiled_traffic(f"{dom}<{args.tile}>")
         if tb is not None:
             roof["traffic"] = tb
