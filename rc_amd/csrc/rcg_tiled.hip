@@ -24,10 +24,10 @@
 //     are summed and stored to HBM with plain stores when the tile has a
 //     single item, with FP32 atomics (non-zero cells only) otherwise;
 //  5. degridding: the workgroup stages its grid tile in LDS, each wavefront
-//     forms the partial sum of one pair over the intersection and reduces it
-//     across lanes; points whose window lies in one tile are stored, split
-//     points leave one partial per tile, summed in a fixed order by
-//     k_degrid_reduce.
+//     forms per-lane partial sums of one pair over the intersection and
+//     reduces them across lanes four pairs at a time; points whose window
+//     lies in one tile are stored, split points leave one partial per tile,
+//     summed in a fixed order by k_degrid_reduce.
 #include <algorithm>
 #include <climits>
 
