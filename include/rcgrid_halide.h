@@ -77,6 +77,25 @@ int kern_degrid_gpu1(const double scale, const int32_t grid_size,
                      const int32_t margin_size, buffer_t *gcf, buffer_t *uvg,
                      buffer_t *vis, buffer_t *vis_out);
 
+/* Replaces nvGridder (MS6/kernel/nvidia/gridder/halide_api.cu:4-29,
+ * grid_gpu.cu:37-243, binsort.cpp:6-32), with its own conventions:
+ *   vis  host array of extent[1] points, 5 contiguous doubles each
+ *        (x, y, z, re, im); sorted IN PLACE by 8-pixel block of the unscaled
+ *        coordinates (block key (gy*2055)/8 + gx with gx = floor(x)/8 in C
+ *        integer arithmetic), then by 8(x - floor x) + (y - floor y); equal
+ *        keys keep their order;
+ *   gcf  host array of 64 16x16 GCFs, [8 sub_y + sub_x][b][a] complex
+ *        doubles;
+ *   uvg  host array of grid_size x grid_size complex doubles, x fast,
+ *        overwritten with the gridded sum;
+ *   each point adds vis * gcf[over][b][a] at pixel (floor(x scale) + a - 7,
+ *   floor(y scale) + b - 7) where that lies in the grid; the oversampling
+ *   offset is floor(8 (xs - floor xs)), likewise y.
+ * `dev` and the dirty flags are cleared as the reference does.  Returns 0,
+ * or -1 with rcg_last_error() set (the reference has no error path). */
+int nvGridder(const double scale, const int32_t grid_size, buffer_t *vis,
+              buffer_t *gcf, buffer_t *uvg);
+
 #ifdef __cplusplus
 }
 #endif

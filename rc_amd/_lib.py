@@ -122,6 +122,7 @@ SIGNATURES: dict[str, tuple] = {
     "kern_degrid": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]),
     "kern_scatter_gpu1": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp]),
     "kern_degrid_gpu1": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]),
+    "nvGridder": (C.c_int, [C.c_double, C.c_int32, vp, vp, vp]),
     # GCF generation (device)
     "rcg_gcf_gen_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_gcf_gen_layer": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

@@ -12,8 +12,13 @@
 // GPU structure: one wavefront per point, lanes over the S*S window
 // (row-major in (y, x)); gridding adds with FP64 atomics, degridding reduces
 // across the wavefront and subtracts.
+//
+// nvGridder (MS6/kernel/nvidia/gridder/halide_api.cu:4-29) is the same
+// scatter with that gridder's own conventions (see nv_scatter below).
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <vector>
 
 #include "rcg_internal.h"
 #include "../../include/rcgrid_halide.h"
@@ -313,8 +318,192 @@ int degrid(double scale, int grid_size, int margin, buffer_t *gcf, buffer_t *uvg
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// nvGridder: MS6/kernel/nvidia/gridder (grid_gpu.cu:37-51 vis2ints,
+// 83-155 grid_kernel_gather, 159-243 host side; binsort.cpp:6-32).
+//   vis   5 doubles per point (x, y, z, re, im), read from `host` as a
+//         contiguous array of extent[1] points, and sorted there in place;
+//   gcf   one 16 x 16 GCF per oversampling offset, [8 sub_y + sub_x][b][a]
+//         complex doubles, read from `host`;
+//   uvg   grid_size x grid_size complex doubles, x (from u) the fast axis,
+//         written (not added to) from `host` on;
+//   pixel xs = x * scale, main_x = floor(xs), sub_x = floor(8 (xs - main_x)),
+//         likewise y; no shift to the grid centre;
+//   taps  pixel (main_x + a - 7, main_y + b - 7) += vis * gcf[over][b][a] for
+//         a, b in [0, 16), where it lies inside the grid.
+// The reference gathers per pixel over visibilities found through
+// bookmarks of its sorted order; for non-negative pixel coordinates and
+// consistent sort and bookmark blocks (scale 1, grids up to 2048) that is
+// this sum.  For negative coordinates it indexes the GCF with a negative
+// sub-pixel offset (C division and remainder truncate toward zero) and
+// reads outside the table; here floor is used throughout.
+
+constexpr int NV_GCF = 16;
+constexpr int NV_TAPS = NV_GCF * NV_GCF;
+
+__global__ void k_nv_zero(double *__restrict__ p, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_nv_scatter(const double *__restrict__ vis, long long n, const double2 *__restrict__ gcf,
+             double *__restrict__ out, double scale, int img)
+{
+#pragma clang fp contract(off)
+  const long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (t >= n) return;
+  const double *r = vis + t * 5;
+  const double xs = r[0] * scale, ys = r[1] * scale;
+  const double fx = floor(xs), fy = floor(ys);
+  // no tap of a point outside these bounds lands in the grid (also NaN)
+  if (!(fx >= -8.0 && fx <= img + 6.0 && fy >= -8.0 && fy <= img + 6.0)) return;
+  const int mx = (int)fx, my = (int)fy;
+  const int sx = (int)floor(H_OVER * (xs - fx)), sy = (int)floor(H_OVER * (ys - fy));
+  const double2 *g = gcf + (H_OVER * sy + sx) * NV_TAPS;
+  const double vr = r[3], vi = r[4];
+  for (int k = lane; k < NV_TAPS; k += 64) {
+    const int px = mx + (k & (NV_GCF - 1)) - 7, py = my + k / NV_GCF - 7;
+    if (px < 0 || px >= img || py < 0 || py >= img) continue;
+    const double2 c = g[k];
+    double *o = out + 2 * ((long long)py * img + px);
+    unsafeAtomicAdd(o, vr * c.x - vi * c.y);
+    unsafeAtomicAdd(o + 1, vr * c.y + c.x * vi);
+  }
+}
+
+// The reference's in-place host sort (binsort.cpp:6-32): by 8-pixel block
+// of the unscaled coordinates, block key (gy * 2055) / 8 + gx in C integer
+// arithmetic (gx = floor(x) / 8 truncated, IMG_SIZE 2048), then by
+// 8 (x - floor x) + (y - floor y).  Its comparator is not a strict weak
+// order on equal keys (std::sort leaves their order unspecified); equal
+// keys keep their input order here.
+struct NvKey {
+  long long blk;
+  double sub;
+};
+
+int sat_int(double v)
+{
+  if (!(v == v)) return 0;
+  return v < (double)INT_MIN ? INT_MIN : v > (double)INT_MAX ? INT_MAX : (int)v;
+}
+
+NvKey nv_key(const double *rec)
+{
+  const int bs = NV_GCF / 2;
+  const int mx = sat_int(std::floor(rec[0])), my = sat_int(std::floor(rec[1]));
+  const long long gx = mx / bs, gy = my / bs;
+  NvKey k;
+  k.blk = gy * (2048 + bs - 1) / bs + gx;
+  k.sub = H_OVER * (rec[0] - mx) + (rec[1] - my);
+  return k;
+}
+
+void nv_sort(double *vis, long long n)
+{
+  std::vector<long long> idx(n);
+  std::vector<NvKey> key(n);
+  for (long long i = 0; i < n; i++) {
+    idx[i] = i;
+    key[i] = nv_key(vis + 5 * i);
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](long long a, long long b) {
+    return key[a].blk < key[b].blk || (key[a].blk == key[b].blk && key[a].sub < key[b].sub);
+  });
+  std::vector<double> tmp(vis, vis + 5 * n);
+  for (long long i = 0; i < n; i++)
+    for (int c = 0; c < 5; c++) vis[5 * i + c] = tmp[5 * idx[i] + c];
+}
+
+int nv_gridder(double scale, int grid_size, buffer_t *vis, buffer_t *gcf, buffer_t *uvg)
+{
+  clear_error();
+  const char *who = "nvGridder";
+  if (!vis || !gcf || !uvg || grid_size < 1) {
+    set_error(RCG_ERR_ARG, "%s: null buffer or grid size < 1", who);
+    return -1;
+  }
+  const long long n = vis->extent[1];
+  const long long cells = (long long)grid_size * grid_size;
+  if (vis->elem_size != 8 || n < 0 || (n > 0 && !vis->host) || vis->extent[0] != 5 ||
+      vis->stride[0] != 1 || (n > 1 && vis->stride[1] != 5)) {
+    set_error(RCG_ERR_ARG, "%s: vis must be contiguous doubles of extent {5, n}", who);
+    return -1;
+  }
+  if (gcf->elem_size != 8 || !gcf->host || span_of(gcf) < 2LL * H_OVER * H_OVER * NV_TAPS) {
+    set_error(RCG_ERR_ARG, "%s: gcf must hold 64 16x16 complex doubles in host memory", who);
+    return -1;
+  }
+  if (uvg->elem_size != 8 || !uvg->host || span_of(uvg) < 2 * cells) {
+    set_error(RCG_ERR_ARG, "%s: uvg must hold grid_size^2 complex doubles in host memory", who);
+    return -1;
+  }
+  double *vh = reinterpret_cast<double *>(vis->host);
+  if (n > 1) nv_sort(vh, n);
+  hipStream_t st = nullptr;
+  double *dv = nullptr, *dout = nullptr;
+  double2 *dg = nullptr;
+  const size_t vbytes = (size_t)std::max(n, 1LL) * 5 * sizeof(double);
+  const size_t gbytes = sizeof(double2) * H_OVER * H_OVER * NV_TAPS;
+  const size_t obytes = sizeof(double) * 2 * (size_t)cells;
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&dv), vbytes) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&dg), gbytes) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&dout), obytes) != hipSuccess) {
+    set_error(RCG_ERR_NOMEM, "%s: device buffers", who);
+    rc = -1;
+  } else {
+    hipError_t e = hipMemcpyAsync(dg, gcf->host, gbytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n > 0)
+      e = hipMemcpyAsync(dv, vh, (size_t)n * 5 * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      const long long nd = 2 * cells;
+      const int zb = (int)std::min<long long>((nd + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_nv_zero, dim3(zb), dim3(256), 0, st, dout, nd);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && n > 0) {
+      const long long blocks = (n * 64 + 255) / 256;
+      hipLaunchKernelGGL(k_nv_scatter, dim3((unsigned)blocks), dim3(256), 0, st, dv, n, dg,
+                         dout, scale, grid_size);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(uvg->host, dout, obytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_error(RCG_ERR_HIP, "%s: kernel or copy failed", who);
+      rc = -1;
+    }
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  if (dv) (void)hipFree(dv);
+  if (dg) (void)hipFree(dg);
+  if (dout) (void)hipFree(dout);
+  if (st) (void)hipStreamDestroy(st);
+  // halide_api.cu:17-26
+  vis->dev = 0;
+  vis->host_dirty = false;
+  gcf->dev = 0;
+  gcf->host_dirty = false;
+  uvg->dev = 0;
+  uvg->host_dirty = false;
+  uvg->dev_dirty = false;
+  return rc;
+}
+
 }  // namespace
 }  // namespace rcg
+
+extern "C" int nvGridder(const double scale, const int32_t grid_size, buffer_t *vis,
+                         buffer_t *gcf, buffer_t *uvg)
+{
+  return rcg::nv_gridder(scale, grid_size, vis, gcf, uvg);
+}
 
 extern "C" int kern_scatter(const double scale, const int32_t grid_size, const int32_t margin_size,
                             buffer_t *vis, buffer_t *gcf, buffer_t *uvg)

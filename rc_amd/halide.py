@@ -1,5 +1,6 @@
 """ctypes view of the MS6 Halide buffer_t ABI (include/rcgrid_halide.h) and
-calls of kern_scatter / kern_degrid / kern_scatter_gpu1 / kern_degrid_gpu1.
+calls of kern_scatter / kern_degrid / kern_scatter_gpu1 / kern_degrid_gpu1 /
+nvGridder.
 
 Array conventions (the MS6 generators' buffer layouts):
   vis  float64 [n, 5]      (u, v, w, re, im)
@@ -73,3 +74,16 @@ def degrid(scale, grid_size, margin, gcf_b, uvg_b, vis_b, out_b, gpu1: bool = Fa
     fn = h.kern_degrid_gpu1 if gpu1 else h.kern_degrid
     return int(fn(scale, grid_size, margin, C.addressof(gcf_b), C.addressof(uvg_b),
                   C.addressof(vis_b), C.addressof(out_b)))
+
+
+def nv_gcf_buffer(gcf: np.ndarray) -> BufferT:
+    """nvGridder's GCF: float64 [64, 16, 16, 2], [8*sub_y + sub_x][b][a]."""
+    assert gcf.dtype == np.float64 and gcf.shape == (64, 16, 16, 2) and gcf.flags.c_contiguous
+    return _buf(gcf, (2, 16, 16, 64), (1, 2, 32, 512))
+
+
+def nv_gridder(scale, grid_size, vis_b, gcf_b, uvg_b) -> int:
+    """nvGridder: sorts the vis buffer's host array in place and overwrites
+    the first grid_size^2 complex cells of the uvg buffer's host array."""
+    return int(_lib.lib().nvGridder(scale, grid_size, C.addressof(vis_b), C.addressof(gcf_b),
+                                    C.addressof(uvg_b)))

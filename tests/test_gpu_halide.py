@@ -93,3 +93,49 @@ def test_unsupported_gcf_size_codes(hip_device):
     dargs = (HB.gcf_buffer(gcf), HB.uvg_buffer(uvg), HB.vis_buffer(vis), HB.vis_buffer(out))
     assert HB.degrid(1.0, N, 0, *dargs) == -555
     assert HB.degrid(1.0, N, 0, *dargs, gpu1=True) == -333
+
+
+# --- nvGridder ---------------------------------------------------------------
+
+def nv_case(n, N, scale, seed):
+    rng = np.random.default_rng(seed)
+    lim = N / scale
+    vis = np.ascontiguousarray(np.column_stack([
+        rng.uniform(0, lim, n), rng.uniform(0, lim, n), rng.standard_normal(n),
+        rng.standard_normal(n), rng.standard_normal(n)]))
+    gcf = np.ascontiguousarray(rng.standard_normal((64, 16, 16, 2)))
+    return vis, gcf
+
+
+@pytest.mark.parametrize("scale,N", [(1.0, 256), (0.5, 200)])
+def test_nv_gridder_vs_oracle(hip_device, scale, N):
+    vis, gcf = nv_case(3000, N, scale, 11)
+    order = H.nv_sort_order(vis)
+    exp = H.nv_grid(vis[order], gcf, scale, N)
+    uvg = np.full((N, N, 2), 7.0)  # overwritten, not added to
+    vb, gb, ub = HB.vis_buffer(vis), HB.nv_gcf_buffer(gcf), HB.uvg_buffer(uvg)
+    vb.dev, ub.dev, ub.dev_dirty = 5, 5, True
+    sorted_exp = vis[order].copy()
+    assert HB.nv_gridder(scale, N, vb, gb, ub) == 0
+    # the host visibility array is sorted in place
+    assert np.array_equal(vis, sorted_exp)
+    assert close(uvg, exp)
+    assert vb.dev == 0 and ub.dev == 0 and not ub.dev_dirty and not vb.host_dirty
+
+
+def test_nv_gridder_edges_and_errors(hip_device):
+    N = 64
+    # points near and beyond the edges, and far outside
+    vis = np.ascontiguousarray(np.array([[0.2, 0.7, 0, 1.0, 0.5], [63.9, 10.0, 0, -1.0, 2.0],
+                                         [70.5, 30.0, 0, 1.0, 1.0], [1e300, 5.0, 0, 1.0, 1.0],
+                                         [30.0, 71.0, 0, 2.0, 0.0]]))
+    gcf = np.ascontiguousarray(np.random.default_rng(2).standard_normal((64, 16, 16, 2)))
+    exp = H.nv_grid(vis, gcf, 1.0, N)  # the far point contributes nothing
+    uvg = np.zeros((N, N, 2))
+    assert HB.nv_gridder(1.0, N, HB.vis_buffer(vis), HB.nv_gcf_buffer(gcf),
+                         HB.uvg_buffer(uvg)) == 0
+    assert close(uvg, exp)
+    # a grid buffer smaller than grid_size^2 cells is refused
+    small = np.zeros((N - 1, N, 2))
+    assert HB.nv_gridder(1.0, N, HB.vis_buffer(vis), HB.nv_gcf_buffer(gcf),
+                         HB.uvg_buffer(small)) == -1

@@ -101,3 +101,82 @@ def test_degrid_is_the_transpose_of_scatter():
     rhs = np.sum((vis[:, 3] + 1j * vis[:, 4]) * dg)
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
     assert np.array_equal(out[:, :3], vis[:, :3])
+
+
+# --- nvGridder (MS6/kernel/nvidia/gridder) ---------------------------------
+
+def nv_delta(over, row, col, val=1.0 + 0.0j):
+    g = np.zeros((64, 16, 16, 2))
+    g[over, row, col] = (val.real, val.imag)
+    return g
+
+
+def test_nv_delta_tap_lands_on_its_pixel():
+    # grid_gpu.cu:131-132 with d_gcf advanced by 16*15/2 - 1 = 119
+    # (grid_gpu.cu:201): pixel (main_x + a, main_y + b) reads flat tap
+    # 256*over + 16*b + a + 119 = [over][b + 7][a + 7]
+    N = 64
+    for scale, (x, y), (sx, sy), (row, col) in ((1.0, (20.3, 30.9), (2, 7), (3, 12)),
+                                                (0.5, (41.0, 61.5), (4, 6), (15, 0))):
+        vis = np.array([[x, y, 0.0, 2.0, -1.0]])
+        g = nv_delta(8 * sy + sx, row, col)
+        out = H.nv_grid(vis, g, scale, N)
+        mx, my = int(np.floor(x * scale)), int(np.floor(y * scale))
+        nz = np.argwhere(out[..., 0] != 0)
+        assert nz.tolist() == [[my + row - 7, mx + col - 7]]
+        assert tuple(out[my + row - 7, mx + col - 7]) == (2.0, -1.0)
+    # the sub-pixel offsets of those points, by hand
+    assert H.nv_pixel(20.3, 30.9, 1.0) == (20, 30, 8 * 7 + 2)
+    assert H.nv_pixel(41.0, 61.5, 0.5) == (20, 30, 8 * 6 + 4)
+
+
+def test_nv_taps_outside_the_grid_are_dropped():
+    N = 32
+    vis = np.array([[2.1, 2.1, 0.0, 1.0, 0.0]])
+    g = np.ones((64, 16, 16, 2))
+    out = H.nv_grid(vis, g, 1.0, N)
+    # pixels 2 + a - 7 >= 0 need a >= 5: 11 columns and 11 rows
+    assert np.count_nonzero(out[..., 0]) == 11 * 11
+    assert out[0, 0, 0] == 1.0 and out[2 + 8, 2 + 8, 0] == 1.0
+
+
+def test_nv_sort_order_keys():
+    # block key (gy * 2055) / 8 + gx on unscaled floor coordinates, C
+    # division: x = -3.5 has floor -4 and -4 / 8 = 0 in C (block 0, not -1),
+    # so it follows x = 1.0 of the same block, whose sub-key is smaller
+    vis = np.array([[-3.5, 0.25, 0, 0, 0],    # block 0, sub 8*0.5 + 0.25 = 4.25
+                    [1.0, 0.5, 0, 0, 0],      # block 0, sub 0.5
+                    [0.5, 8.5, 0, 0, 0],      # gy 1: block 2055 / 8 = 256
+                    [9.0, 0.0, 0, 0, 0],      # block 1, sub 0
+                    [2060.5, 0.5, 0, 0, 0]])  # gx 257: block 257, after gy = 1
+    assert H.nv_sort_order(vis).tolist() == [1, 0, 3, 2, 4]
+    # equal keys keep their input order
+    same = np.array([[1.25, 2.5, 0, 0, k] for k in range(4)], float)
+    assert H.nv_sort_order(same).tolist() == [0, 1, 2, 3]
+
+
+def test_nv_grid_matches_the_reference_gather_loops():
+    """The pixel-centric gather of grid_gpu.cu:119-139 written with its own
+    index arithmetic (flat tap 256*over + 16*b + a + 119, a and b in
+    (-8, 8]) on tiny inputs, against the oracle's scatter."""
+    rng = np.random.default_rng(3)
+    N, n = 24, 20
+    vis = np.column_stack([rng.uniform(0, N, n), rng.uniform(0, N, n), np.zeros(n),
+                           rng.standard_normal(n), rng.standard_normal(n)])
+    gcf = rng.standard_normal((64, 16, 16, 2))
+    flat = (gcf[..., 0] + 1j * gcf[..., 1]).reshape(-1)
+    exp = np.zeros((N, N), complex)
+    for this_y in range(N):
+        for this_x in range(N):
+            for x, y, _, re, im in vis:
+                ix = int(np.floor(x)) * 8 + int(np.floor(8 * (x - np.floor(x))))
+                iy = int(np.floor(y)) * 8 + int(np.floor(8 * (y - np.floor(y))))
+                main_x, sub_x = ix // 8, ix % 8
+                main_y, sub_y = iy // 8, iy % 8
+                a, b = this_x - main_x, this_y - main_y
+                if not (-8 < a <= 8 and -8 < b <= 8):
+                    continue
+                exp[this_y, this_x] += (re + 1j * im) * flat[256 * (8 * sub_y + sub_x) +
+                                                            16 * b + a + 119]
+    got = H.nv_grid(vis, gcf, 1.0, N)
+    assert np.allclose(got[..., 0] + 1j * got[..., 1], exp, rtol=0, atol=1e-12)
