@@ -33,6 +33,7 @@
 #define RCGRID_HALIDE_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -95,6 +96,24 @@ int kern_degrid_gpu1(const double scale, const int32_t grid_size,
  * or -1 with rcg_last_error() set (the reference has no error path). */
 int nvGridder(const double scale, const int32_t grid_size, buffer_t *vis,
               buffer_t *gcf, buffer_t *uvg);
+
+/* The other exports of the same gridder (grid_gpu.cuh:27-43; binsort.cpp:
+ * 33-34), on raw host pointers (the reference's double3 / combined /
+ * CmplxType records are 3, 5 and 2 doubles, passed here as double *):
+ *   bin_sort3 / bin_sort5  nvGridder's in-place sort of npts records of 3
+ *                          (x, y, z) or 5 (x, y, z, re, im) doubles;
+ *   gridGPUc  grids npts 5-double records (no sort) into out, img_dim^2
+ *             complex doubles, overwritten; the reference's gather assumes
+ *             sorted input, this grids every point;
+ *   gridGPUs  the same with positions (3 doubles per point) and
+ *             visibilities (2 doubles per point) in separate arrays.
+ * Errors are reported through rcg_last_error() (the reference has none). */
+void bin_sort3(double *in, int npts);
+void bin_sort5(double *in, int npts);
+void gridGPUc(double scale, double *out, double *in_c, size_t npts, size_t img_dim,
+              double *gcf);
+void gridGPUs(double scale, double *out, double *in, double *in_vals, size_t npts,
+              size_t img_dim, double *gcf);
 
 #ifdef __cplusplus
 }

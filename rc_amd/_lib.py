@@ -123,6 +123,10 @@ SIGNATURES: dict[str, tuple] = {
     "kern_scatter_gpu1": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp]),
     "kern_degrid_gpu1": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]),
     "nvGridder": (C.c_int, [C.c_double, C.c_int32, vp, vp, vp]),
+    "bin_sort3": (None, [vp, C.c_int]),
+    "bin_sort5": (None, [vp, C.c_int]),
+    "gridGPUc": (None, [C.c_double, vp, vp, C.c_size_t, C.c_size_t, vp]),
+    "gridGPUs": (None, [C.c_double, vp, vp, vp, C.c_size_t, C.c_size_t, vp]),
     # GCF generation (device)
     "rcg_gcf_gen_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_gcf_gen_layer": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

@@ -348,23 +348,25 @@ __global__ void k_nv_zero(double *__restrict__ p, long long n)
     p[i] = 0.0;
 }
 
+// pos: x, y of point t at pos[t * ps], pos[t * ps + 1]; val: re, im at
+// val[t * vs], val[t * vs + 1]
 __global__ void __launch_bounds__(256)
-k_nv_scatter(const double *__restrict__ vis, long long n, const double2 *__restrict__ gcf,
-             double *__restrict__ out, double scale, int img)
+k_nv_scatter(const double *__restrict__ pos, int ps, const double *__restrict__ val, int vs,
+             long long n, const double2 *__restrict__ gcf, double *__restrict__ out,
+             double scale, int img)
 {
 #pragma clang fp contract(off)
   const long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (t >= n) return;
-  const double *r = vis + t * 5;
-  const double xs = r[0] * scale, ys = r[1] * scale;
+  const double xs = pos[t * ps] * scale, ys = pos[t * ps + 1] * scale;
   const double fx = floor(xs), fy = floor(ys);
   // no tap of a point outside these bounds lands in the grid (also NaN)
   if (!(fx >= -8.0 && fx <= img + 6.0 && fy >= -8.0 && fy <= img + 6.0)) return;
   const int mx = (int)fx, my = (int)fy;
   const int sx = (int)floor(H_OVER * (xs - fx)), sy = (int)floor(H_OVER * (ys - fy));
   const double2 *g = gcf + (H_OVER * sy + sx) * NV_TAPS;
-  const double vr = r[3], vi = r[4];
+  const double vr = val[t * vs], vi = val[t * vs + 1];
   for (int k = lane; k < NV_TAPS; k += 64) {
     const int px = mx + (k & (NV_GCF - 1)) - 7, py = my + k / NV_GCF - 7;
     if (px < 0 || px >= img || py < 0 || py >= img) continue;
@@ -403,20 +405,80 @@ NvKey nv_key(const double *rec)
   return k;
 }
 
-void nv_sort(double *vis, long long n)
+// records of `rec` doubles, (x, y) first
+void nv_sort(double *recs, long long n, int rec)
 {
+  if (n < 2) return;
   std::vector<long long> idx(n);
   std::vector<NvKey> key(n);
   for (long long i = 0; i < n; i++) {
     idx[i] = i;
-    key[i] = nv_key(vis + 5 * i);
+    key[i] = nv_key(recs + rec * i);
   }
   std::stable_sort(idx.begin(), idx.end(), [&](long long a, long long b) {
     return key[a].blk < key[b].blk || (key[a].blk == key[b].blk && key[a].sub < key[b].sub);
   });
-  std::vector<double> tmp(vis, vis + 5 * n);
+  std::vector<double> tmp(recs, recs + rec * n);
   for (long long i = 0; i < n; i++)
-    for (int c = 0; c < 5; c++) vis[5 * i + c] = tmp[5 * idx[i] + c];
+    for (int c = 0; c < rec; c++) recs[rec * i + c] = tmp[rec * idx[i] + c];
+}
+
+// Grids n points into out (img x img complex doubles, host, overwritten).
+// pos / val: host arrays of n records of ps / vs doubles, (x, y) and
+// (re, im) first.  shared: 5-double records holding both, val unused.
+int nv_grid_host(const char *who, double scale, double *out, const double *pos, int ps,
+                 const double *val, int vs, bool shared, long long n, int img,
+                 const double *gcf)
+{
+  const long long cells = (long long)img * img;
+  hipStream_t st = nullptr;
+  double *dp = nullptr, *dv = nullptr, *dout = nullptr;
+  double2 *dg = nullptr;
+  const size_t pbytes = (size_t)std::max(n, 1LL) * ps * sizeof(double);
+  const size_t vbytes = (size_t)std::max(n, 1LL) * vs * sizeof(double);
+  const size_t gbytes = sizeof(double2) * H_OVER * H_OVER * NV_TAPS;
+  const size_t obytes = sizeof(double) * 2 * (size_t)cells;
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&dp), pbytes) != hipSuccess ||
+      (!shared && hipMalloc(reinterpret_cast<void **>(&dv), vbytes) != hipSuccess) ||
+      hipMalloc(reinterpret_cast<void **>(&dg), gbytes) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&dout), obytes) != hipSuccess) {
+    set_error(RCG_ERR_NOMEM, "%s: device buffers", who);
+    rc = -1;
+  } else {
+    hipError_t e = hipMemcpyAsync(dg, gcf, gbytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n > 0)
+      e = hipMemcpyAsync(dp, pos, (size_t)n * ps * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n > 0 && !shared)
+      e = hipMemcpyAsync(dv, val, (size_t)n * vs * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      const long long nd = 2 * cells;
+      const int zb = (int)std::min<long long>((nd + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_nv_zero, dim3(zb), dim3(256), 0, st, dout, nd);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && n > 0) {
+      const long long blocks = (n * 64 + 255) / 256;
+      const double *vals = shared ? dp + 3 : dv;
+      hipLaunchKernelGGL(k_nv_scatter, dim3((unsigned)blocks), dim3(256), 0, st, dp, ps, vals,
+                         vs, n, dg, dout, scale, img);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, obytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_error(RCG_ERR_HIP, "%s: kernel or copy failed", who);
+      rc = -1;
+    }
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  if (dp) (void)hipFree(dp);
+  if (dv) (void)hipFree(dv);
+  if (dg) (void)hipFree(dg);
+  if (dout) (void)hipFree(dout);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
 }
 
 int nv_gridder(double scale, int grid_size, buffer_t *vis, buffer_t *gcf, buffer_t *uvg)
@@ -443,48 +505,10 @@ int nv_gridder(double scale, int grid_size, buffer_t *vis, buffer_t *gcf, buffer
     return -1;
   }
   double *vh = reinterpret_cast<double *>(vis->host);
-  if (n > 1) nv_sort(vh, n);
-  hipStream_t st = nullptr;
-  double *dv = nullptr, *dout = nullptr;
-  double2 *dg = nullptr;
-  const size_t vbytes = (size_t)std::max(n, 1LL) * 5 * sizeof(double);
-  const size_t gbytes = sizeof(double2) * H_OVER * H_OVER * NV_TAPS;
-  const size_t obytes = sizeof(double) * 2 * (size_t)cells;
-  int rc = 0;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&dv), vbytes) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&dg), gbytes) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&dout), obytes) != hipSuccess) {
-    set_error(RCG_ERR_NOMEM, "%s: device buffers", who);
-    rc = -1;
-  } else {
-    hipError_t e = hipMemcpyAsync(dg, gcf->host, gbytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && n > 0)
-      e = hipMemcpyAsync(dv, vh, (size_t)n * 5 * sizeof(double), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-      const long long nd = 2 * cells;
-      const int zb = (int)std::min<long long>((nd + 255) / 256, 65536);
-      hipLaunchKernelGGL(k_nv_zero, dim3(zb), dim3(256), 0, st, dout, nd);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess && n > 0) {
-      const long long blocks = (n * 64 + 255) / 256;
-      hipLaunchKernelGGL(k_nv_scatter, dim3((unsigned)blocks), dim3(256), 0, st, dv, n, dg,
-                         dout, scale, grid_size);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(uvg->host, dout, obytes, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-      set_error(RCG_ERR_HIP, "%s: kernel or copy failed", who);
-      rc = -1;
-    }
-  }
-  if (st) (void)hipStreamSynchronize(st);
-  if (dv) (void)hipFree(dv);
-  if (dg) (void)hipFree(dg);
-  if (dout) (void)hipFree(dout);
-  if (st) (void)hipStreamDestroy(st);
+  nv_sort(vh, n, 5);
+  const int rc = nv_grid_host(who, scale, reinterpret_cast<double *>(uvg->host), vh, 5, nullptr,
+                              5, true, n, grid_size,
+                              reinterpret_cast<const double *>(gcf->host));
   // halide_api.cu:17-26
   vis->dev = 0;
   vis->host_dirty = false;
@@ -496,6 +520,17 @@ int nv_gridder(double scale, int grid_size, buffer_t *vis, buffer_t *gcf, buffer
   return rc;
 }
 
+bool nv_raw_args(const char *who, const void *out, const void *in, size_t npts, size_t img,
+                 const void *gcf)
+{
+  if (!out || !gcf || (npts > 0 && !in) || img < 1 || img > 46340 ||
+      npts > (size_t)LLONG_MAX / 64) {
+    set_error(RCG_ERR_ARG, "%s: null pointer or size out of range", who);
+    return false;
+  }
+  return true;
+}
+
 }  // namespace
 }  // namespace rcg
 
@@ -503,6 +538,39 @@ extern "C" int nvGridder(const double scale, const int32_t grid_size, buffer_t *
                          buffer_t *gcf, buffer_t *uvg)
 {
   return rcg::nv_gridder(scale, grid_size, vis, gcf, uvg);
+}
+
+extern "C" void bin_sort3(double *in, int npts)
+{
+  rcg::clear_error();
+  if (in && npts > 1) rcg::nv_sort(in, npts, 3);
+}
+
+extern "C" void bin_sort5(double *in, int npts)
+{
+  rcg::clear_error();
+  if (in && npts > 1) rcg::nv_sort(in, npts, 5);
+}
+
+extern "C" void gridGPUc(double scale, double *out, double *in_c, size_t npts, size_t img_dim,
+                         double *gcf)
+{
+  rcg::clear_error();
+  if (!rcg::nv_raw_args("gridGPUc", out, in_c, npts, img_dim, gcf)) return;
+  (void)rcg::nv_grid_host("gridGPUc", scale, out, in_c, 5, nullptr, 5, true, (long long)npts,
+                          (int)img_dim, gcf);
+}
+
+extern "C" void gridGPUs(double scale, double *out, double *in, double *in_vals, size_t npts,
+                         size_t img_dim, double *gcf)
+{
+  rcg::clear_error();
+  if (!rcg::nv_raw_args("gridGPUs", out, in, npts, img_dim, gcf) || (npts > 0 && !in_vals)) {
+    if (npts > 0 && !in_vals) rcg::set_error(RCG_ERR_ARG, "gridGPUs: null in_vals");
+    return;
+  }
+  (void)rcg::nv_grid_host("gridGPUs", scale, out, in, 3, in_vals, 2, false, (long long)npts,
+                          (int)img_dim, gcf);
 }
 
 extern "C" int kern_scatter(const double scale, const int32_t grid_size, const int32_t margin_size,

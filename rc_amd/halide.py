@@ -87,3 +87,36 @@ def nv_gridder(scale, grid_size, vis_b, gcf_b, uvg_b) -> int:
     the first grid_size^2 complex cells of the uvg buffer's host array."""
     return int(_lib.lib().nvGridder(scale, grid_size, C.addressof(vis_b), C.addressof(gcf_b),
                                     C.addressof(uvg_b)))
+
+
+def bin_sort(recs: np.ndarray) -> None:
+    """bin_sort3 / bin_sort5: nvGridder's in-place sort of float64 [n, 3] or
+    [n, 5] records."""
+    assert recs.dtype == np.float64 and recs.ndim == 2 and recs.shape[1] in (3, 5)
+    assert recs.flags.c_contiguous
+    fn = _lib.lib().bin_sort3 if recs.shape[1] == 3 else _lib.lib().bin_sort5
+    fn(recs.ctypes.data, recs.shape[0])
+
+
+def grid_gpu(scale: float, gcf: np.ndarray, img_dim: int, combined: np.ndarray | None = None,
+             pos: np.ndarray | None = None, vals: np.ndarray | None = None) -> np.ndarray:
+    """gridGPUc (combined [n, 5]) or gridGPUs (pos [n, 3], vals [n, 2]):
+    returns the img_dim x img_dim x 2 grid (x fast)."""
+    assert gcf.dtype == np.float64 and gcf.shape == (64, 16, 16, 2) and gcf.flags.c_contiguous
+    out = np.zeros((img_dim, img_dim, 2))
+    h = _lib.lib()
+    if combined is not None:
+        assert combined.dtype == np.float64 and combined.shape[1] == 5
+        assert combined.flags.c_contiguous
+        h.gridGPUc(scale, out.ctypes.data, combined.ctypes.data, combined.shape[0], img_dim,
+                   gcf.ctypes.data)
+    else:
+        assert pos.dtype == np.float64 and pos.shape[1] == 3 and pos.flags.c_contiguous
+        assert vals.dtype == np.float64 and vals.shape == (pos.shape[0], 2)
+        assert vals.flags.c_contiguous
+        h.gridGPUs(scale, out.ctypes.data, pos.ctypes.data, vals.ctypes.data, pos.shape[0],
+                   img_dim, gcf.ctypes.data)
+    code, msg = _lib.last_error()
+    if code:
+        raise _lib.RcgError(f"gridGPU failed: {msg}")
+    return out

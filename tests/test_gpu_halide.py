@@ -139,3 +139,23 @@ def test_nv_gridder_edges_and_errors(hip_device):
     small = np.zeros((N - 1, N, 2))
     assert HB.nv_gridder(1.0, N, HB.vis_buffer(vis), HB.nv_gcf_buffer(gcf),
                          HB.uvg_buffer(small)) == -1
+
+
+def test_bin_sort_and_raw_gridders(hip_device):
+    """bin_sort3 / bin_sort5 and gridGPUc / gridGPUs (grid_gpu.cuh:27-43)."""
+    vis, gcf = nv_case(2000, 128, 1.0, 4)
+    order = H.nv_sort_order(vis)
+    five = vis.copy()
+    HB.bin_sort(five)
+    assert np.array_equal(five, vis[order])
+    three = np.ascontiguousarray(vis[:, :3])
+    HB.bin_sort(three)
+    assert np.array_equal(three, vis[order][:, :3])
+    exp = H.nv_grid(vis, gcf, 1.0, 128)
+    # gridGPUc does not sort: the caller's array is left as it is
+    c = vis.copy()
+    assert close(HB.grid_gpu(1.0, gcf, 128, combined=c), exp)
+    assert np.array_equal(c, vis)
+    pos = np.ascontiguousarray(vis[:, :3])
+    vals = np.ascontiguousarray(vis[:, 3:])
+    assert close(HB.grid_gpu(1.0, gcf, 128, pos=pos, vals=vals), exp)
