@@ -115,6 +115,40 @@ void gridGPUc(double scale, double *out, double *in_c, size_t npts, size_t img_d
 void gridGPUs(double scale, double *out, double *in, double *in_vals, size_t npts,
               size_t img_dim, double *gcf);
 
+/* The MS6 imaging-glue kernels (Halide generators in
+ * MS6/kernel/cpu/gridding; Haskell bindings MS6/programs/Kernel/
+ * {Gridder,Facet,Cleaning}.hs).  Each is computed over its output buffer's
+ * region (dimension i covers [min[i], min[i] + extent[i])) on host buffers;
+ * the computation runs on the GPU.  Return 0, or -1 with rcg_last_error()
+ * set where Halide would report a constraint or bounds error.
+ *   kern_init        (init.cpp)        grid(c, x, y) = 0;
+ *   kern_image_init  (image_init.cpp)  image(x, y) = 0;
+ *   kern_psf_vis     (psf_vis.cpp:17-20) out = vis with re = 1, im = 0
+ *                    (5 doubles per point, dimension 0 min 0 extent 5);
+ *   kern_rotate      (rotate.cpp:28-68, utils.h:137-158) facet (l, m) =
+ *                    the output's min[2], min[3]; centre (out_lon0 + l
+ *                    out_lon_incr, out_lat0 + m out_lat_incr); R =
+ *                    xyz2uvw(out) uvw2xyz(in); uvw' = uvw, or R^T (inv2x2(R)^T
+ *                    R) uvw when uvproj; vis' = vis exp(2 pi i (w - R w).uvw);
+ *   kern_detile      (detile.cpp:19-30) out(c, x, y) += in(c, x, y) over the
+ *                    overlap of both buffers' x and y ranges;
+ *   kern_defacet     (defacet.cpp:12-20) image(clamp(x + l), clamp(y + m)) =
+ *                    facet(x, y, l, m), x fastest, the last write winning;
+ *   kern_image_sum   (image_sum.cpp:12-20) sum(clamp(x), clamp(y)) +=
+ *                    image(x, y);
+ * where clamp() limits to the output's dimension-0 range [left, right], for
+ * y as well (the reference clamps y with the x bounds). */
+int kern_init(buffer_t *grid);
+int kern_image_init(buffer_t *image);
+int kern_psf_vis(buffer_t *vis, buffer_t *out);
+int kern_rotate(const double in_lon, const double in_lat, const double out_lon0,
+                const double out_lat0, const double out_lon_incr,
+                const double out_lat_incr, const int32_t uvproj, buffer_t *vis,
+                buffer_t *rot);
+int kern_detile(buffer_t *in_grid, buffer_t *out_grid);
+int kern_defacet(buffer_t *facet, buffer_t *image);
+int kern_image_sum(buffer_t *image, buffer_t *sum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -133,3 +133,102 @@ def nv_grid(vis: np.ndarray, gcf: np.ndarray, scale: float, grid_size: int) -> n
                 if 0 <= px < grid_size:
                     out[py, px] += (re + 1j * im) * g[over, b, a]
     return np.stack([out.real, out.imag], -1)
+
+
+# --- MS6 imaging glue (MS6/kernel/cpu/gridding) --------------------------------
+# Arrays are numpy views of Halide buffers with the last index fastest:
+# a buffer of dimensions (x0, x1, ...) is an array [..., x1, x0]; `mn` gives
+# the Halide min of each dimension (x0 first).
+
+def psf_vis(vis: np.ndarray) -> np.ndarray:
+    """psf_vis.cpp:17-20: re = 1, im = 0, u v w kept (vis [n, 5])."""
+    out = vis.copy()
+    out[:, 3] = 1.0
+    out[:, 4] = 0.0
+    return out
+
+
+def _rx(p):
+    s, c = np.sin(p), np.cos(p)
+    return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+
+
+def _rz(p):
+    s, c = np.sin(p), np.cos(p)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+
+
+def rotation(in_lon, in_lat, out_lon, out_lat, uvproj):
+    """rotate.cpp:33-52 with utils.h:137-158: (R, mtx, pos) where
+    R = xyz2uvw(out) uvw2xyz(in), mtx the uvw transform (identity unless
+    uvproj) and pos = w - R w."""
+    pi = 4 * np.arctan(1.0)
+    xyz2uvw = _rx(-(pi / 2) + out_lat) @ _rz(-(pi / 2) - out_lon)
+    uvw2xyz = _rz((pi / 2) + in_lon) @ _rx((pi / 2) - in_lat)
+    R = xyz2uvw @ uvw2xyz
+    if uvproj:
+        det = R[0, 0] * R[1, 1] - R[0, 1] * R[1, 0]
+        inv = np.array([[R[1, 1] / det, -R[0, 1] / det, 0], [-R[1, 0] / det, R[0, 0] / det, 0],
+                        [0, 0, 1]])
+        mtx = R.T @ (inv.T @ R)
+    else:
+        mtx = np.eye(3)
+    pos = np.array([0.0, 0.0, 1.0]) - R @ np.array([0.0, 0.0, 1.0])
+    return R, mtx, pos
+
+
+def rotate(vis: np.ndarray, in_lon, in_lat, out_lon0, out_lat0, lon_incr, lat_incr, uvproj,
+           l_min: int, m_min: int) -> np.ndarray:
+    """rotate.cpp:28-68: the facet at (l_min, m_min) (the output buffer's
+    min[2], min[3]) has centre (out_lon0 + l_min lon_incr, out_lat0 + m_min
+    lat_incr); uvw' = mtx uvw, vis' = vis exp(2 pi i pos.uvw)."""
+    out_lon = out_lon0 + l_min * lon_incr
+    out_lat = out_lat0 + m_min * lat_incr
+    _, mtx, pos = rotation(in_lon, in_lat, out_lon, out_lat, uvproj)
+    uvw = vis[:, :3]
+    out = np.empty_like(vis)
+    out[:, :3] = uvw @ mtx.T
+    phi = 2 * (4 * np.arctan(1.0)) * (uvw @ pos)
+    v = (vis[:, 3] + 1j * vis[:, 4]) * (np.cos(phi) + 1j * np.sin(phi))
+    out[:, 3], out[:, 4] = v.real, v.imag
+    return out
+
+
+def detile(in_grid: np.ndarray, in_mn, out_grid: np.ndarray, out_mn) -> None:
+    """detile.cpp:19-30: out += in over the overlap of the (x, y) ranges;
+    grids [Y, X, 2], mn = (x, y) of the first cell."""
+    x0, y0 = max(in_mn[0], out_mn[0]), max(in_mn[1], out_mn[1])
+    x1 = min(in_mn[0] + in_grid.shape[1], out_mn[0] + out_grid.shape[1])
+    y1 = min(in_mn[1] + in_grid.shape[0], out_mn[1] + out_grid.shape[0])
+    if x1 <= x0 or y1 <= y0:
+        return
+    out_grid[y0 - out_mn[1]:y1 - out_mn[1], x0 - out_mn[0]:x1 - out_mn[0]] += \
+        in_grid[y0 - in_mn[1]:y1 - in_mn[1], x0 - in_mn[0]:x1 - in_mn[0]]
+
+
+def defacet(facet: np.ndarray, f_mn, image: np.ndarray, i_mn) -> None:
+    """defacet.cpp:12-20: image(clamp(x + l, left, right), clamp(y + m, left,
+    right)) = facet(x, y, l, m) in Halide order (x fastest), so the last
+    write wins; left / right are the image's x bounds, used for y as well.
+    facet [M, L, Y, X], f_mn = (x, y, l, m); image [Y, X], i_mn = (x, y)."""
+    left, right = i_mn[0], i_mn[0] + image.shape[1] - 1
+    M, L, Y, X = facet.shape
+    for m in range(f_mn[3], f_mn[3] + M):
+        for l in range(f_mn[2], f_mn[2] + L):
+            for y in range(f_mn[1], f_mn[1] + Y):
+                for x in range(f_mn[0], f_mn[0] + X):
+                    ix = min(max(x + l, left), right)
+                    iy = min(max(y + m, left), right)
+                    image[iy - i_mn[1], ix - i_mn[0]] = \
+                        facet[m - f_mn[3], l - f_mn[2], y - f_mn[1], x - f_mn[0]]
+
+
+def image_sum(image: np.ndarray, i_mn, out: np.ndarray, o_mn) -> None:
+    """image_sum.cpp:12-20: out(clamp(x, left, right), clamp(y, left, right))
+    += image(x, y); left / right are out's x bounds, used for y as well."""
+    left, right = o_mn[0], o_mn[0] + out.shape[1] - 1
+    Y, X = image.shape
+    for y in range(i_mn[1], i_mn[1] + Y):
+        for x in range(i_mn[0], i_mn[0] + X):
+            ox, oy = min(max(x, left), right), min(max(y, left), right)
+            out[oy - o_mn[1], ox - o_mn[0]] += image[y - i_mn[1], x - i_mn[0]]

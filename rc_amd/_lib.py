@@ -127,6 +127,14 @@ SIGNATURES: dict[str, tuple] = {
     "bin_sort5": (None, [vp, C.c_int]),
     "gridGPUc": (None, [C.c_double, vp, vp, C.c_size_t, C.c_size_t, vp]),
     "gridGPUs": (None, [C.c_double, vp, vp, vp, C.c_size_t, C.c_size_t, vp]),
+    "kern_init": (C.c_int, [vp]),
+    "kern_image_init": (C.c_int, [vp]),
+    "kern_psf_vis": (C.c_int, [vp, vp]),
+    "kern_rotate": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_int32, vp, vp]),
+    "kern_detile": (C.c_int, [vp, vp]),
+    "kern_defacet": (C.c_int, [vp, vp]),
+    "kern_image_sum": (C.c_int, [vp, vp]),
     # GCF generation (device)
     "rcg_gcf_gen_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_gcf_gen_layer": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

@@ -531,6 +531,458 @@ bool nv_raw_args(const char *who, const void *out, const void *in, size_t npts, 
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// MS6 imaging-glue kernels (Halide generators in MS6/kernel/cpu/gridding:
+// init.cpp, image_init.cpp, psf_vis.cpp, rotate.cpp, detile.cpp,
+// defacet.cpp, image_sum.cpp).  A Halide function is computed over its
+// output buffer's region, dimension i covering [min[i], min[i] + extent[i]);
+// element (x0, x1, x2, x3) of a buffer lies at sum_i (x_i - min[i]) *
+// stride[i] elements from its start.  The buffers are in host memory; each
+// call copies the spans it reads to the device and the output back.
+
+struct G4 {
+  int mn[4], ext[4], str[4];
+};
+
+G4 g4_of(const buffer_t *b, int dims)
+{
+  G4 g;
+  for (int i = 0; i < 4; i++) {
+    g.mn[i] = b->min[i];
+    g.ext[i] = i < dims ? b->extent[i] : 1;
+    g.str[i] = b->stride[i];
+  }
+  return g;
+}
+
+__device__ __forceinline__ long long g4_off(const G4 &g, int x0, int x1, int x2, int x3)
+{
+  return (long long)(x0 - g.mn[0]) * g.str[0] + (long long)(x1 - g.mn[1]) * g.str[1] +
+         (long long)(x2 - g.mn[2]) * g.str[2] + (long long)(x3 - g.mn[3]) * g.str[3];
+}
+
+// region index -> coordinates (x0 fastest)
+__device__ __forceinline__ void g4_coord(const G4 &g, long long i, int *x)
+{
+  for (int d = 0; d < 4; d++) {
+    const long long e = g.ext[d];
+    x[d] = g.mn[d] + (int)(i % e);
+    i /= e;
+  }
+}
+
+__global__ void k_g_zero(double *__restrict__ p, G4 g, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int x[4];
+    g4_coord(g, i, x);
+    p[g4_off(g, x[0], x[1], x[2], x[3])] = 0.0;
+  }
+}
+
+// psf_vis.cpp:17-20: out(c, t) = c == re ? 1 : c == im ? 0 : vis(c, t)
+__global__ void k_g_psf_vis(const double *__restrict__ vis, G4 vg, double *__restrict__ out,
+                            G4 og, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int x[4];
+    g4_coord(og, i, x);
+    const double v =
+        x[0] == 3 ? 1.0 : x[0] == 4 ? 0.0 : vis[g4_off(vg, x[0], x[1], vg.mn[2], vg.mn[3])];
+    out[g4_off(og, x[0], x[1], x[2], x[3])] = v;
+  }
+}
+
+// rotate.cpp:48-68: per point, uvw' = mtx uvw and vis' = vis *
+// exp(2 pi i (pos . uvw)), with mtx and pos (= w - R w) precomputed
+struct RotPar {
+  double m[9], pos[3];
+};
+
+__global__ void k_g_rotate(const double *__restrict__ vis, G4 vg, double *__restrict__ out,
+                           G4 og, RotPar rp)
+{
+#pragma clang fp contract(off)
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long k = t0; k < og.ext[1]; k += (long long)gridDim.x * blockDim.x) {
+    const int t = og.mn[1] + (int)k;
+    const double *r = vis + (long long)(t - vg.mn[1]) * vg.str[1] - (long long)vg.mn[0] * vg.str[0];
+    const double u = r[0], v = r[vg.str[0]], w = r[2 * vg.str[0]];
+    const double re = r[3 * vg.str[0]], im = r[4 * vg.str[0]];
+    const double nu = rp.m[0] * u + rp.m[1] * v + rp.m[2] * w;
+    const double nv = rp.m[3] * u + rp.m[4] * v + rp.m[5] * w;
+    const double nw = rp.m[6] * u + rp.m[7] * v + rp.m[8] * w;
+    const double path = rp.pos[0] * u + rp.pos[1] * v + rp.pos[2] * w;
+    const double phi = 2.0 * (4.0 * atan(1.0)) * path;
+    const double c = cos(phi), s = sin(phi);
+    double *o = out + (long long)(t - og.mn[1]) * og.str[1] - (long long)og.mn[0] * og.str[0];
+    const double vals[5] = {nu, nv, nw, c * re - s * im, c * im + s * re};
+    for (int q = og.mn[0]; q < og.mn[0] + og.ext[0]; q++) o[q * og.str[0]] = vals[q];
+  }
+}
+
+// detile.cpp:19-30: out(c, x, y) += in(c, x, y) over the overlap of both
+// buffers' x and y ranges (c in [0, 2))
+__global__ void k_g_detile(const double *__restrict__ in, G4 ig, double *__restrict__ out,
+                           G4 og, int x0, int nx, int y0, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i & 1);
+    const long long p = i >> 1;
+    const int x = x0 + (int)(p % nx), y = y0 + (int)(p / nx);
+    out[g4_off(og, c, x, y, og.mn[3])] += in[g4_off(ig, c, x, y, ig.mn[3])];
+  }
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+// defacet.cpp:12-20: image(clamp(x + l, left, right), clamp(y + m, left,
+// right)) = facet(x, y, l, m), x fastest; the last write in that order
+// wins.  Pass 1 records the largest source index per target pixel, pass 2
+// lets that source write.  left / right are the image's dimension-0 bounds,
+// used for y too, as in the reference.
+__global__ void k_g_defacet_claim(G4 fg, G4 og, int left, int right,
+                                  unsigned long long *__restrict__ win, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int x[4];
+    g4_coord(fg, i, x);
+    const int ix = clampi(x[0] + x[2], left, right), iy = clampi(x[1] + x[3], left, right);
+    const long long p = (long long)(iy - og.mn[1]) * og.ext[0] + (ix - og.mn[0]);
+    atomicMax(&win[p], (unsigned long long)i + 1);
+  }
+}
+
+__global__ void k_g_defacet_write(const double *__restrict__ facet, G4 fg, double *__restrict__ img,
+                                  G4 og, int left, int right,
+                                  const unsigned long long *__restrict__ win, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int x[4];
+    g4_coord(fg, i, x);
+    const int ix = clampi(x[0] + x[2], left, right), iy = clampi(x[1] + x[3], left, right);
+    const long long p = (long long)(iy - og.mn[1]) * og.ext[0] + (ix - og.mn[0]);
+    if (win[p] == (unsigned long long)i + 1)
+      img[g4_off(og, ix, iy, og.mn[2], og.mn[3])] = facet[g4_off(fg, x[0], x[1], x[2], x[3])];
+  }
+}
+
+// image_sum.cpp:12-20: sum(clamp(x, left, right), clamp(y, left, right)) +=
+// image(x, y) over the input image's region (left / right: the output's
+// dimension-0 bounds, for y too)
+__global__ void k_g_image_sum(const double *__restrict__ img, G4 ig, double *__restrict__ sum,
+                              G4 og, int left, int right, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int x[4];
+    g4_coord(ig, i, x);
+    const int ox = clampi(x[0], left, right), oy = clampi(x[1], left, right);
+    unsafeAtomicAdd(&sum[g4_off(og, ox, oy, og.mn[2], og.mn[3])],
+                    img[g4_off(ig, x[0], x[1], x[2], x[3])]);
+  }
+}
+
+long long region_size(const G4 &g)
+{
+  long long n = 1;
+  for (int i = 0; i < 4; i++) n *= g.ext[i];
+  return n;
+}
+
+// elements from the buffer's start to one past its last element (strides
+// positive, offsets measured from min)
+long long span_elems(const buffer_t *b, int dims)
+{
+  long long s = 1;
+  for (int i = 0; i < dims; i++)
+    if (b->extent[i] > 0) s += (long long)(b->extent[i] - 1) * b->stride[i];
+  return s;
+}
+
+bool host_buf(const buffer_t *b, int dims, const char *who, const char *name)
+{
+  bool ok = b && b->host && b->elem_size == 8;
+  for (int i = 0; ok && i < dims; i++) ok = b->extent[i] >= 1 && b->stride[i] >= 1;
+  if (!ok) set_error(RCG_ERR_ARG, "%s: %s must be a host buffer of doubles with %d dimensions", who,
+                     name, dims);
+  return ok;
+}
+
+// Runs `launch` on device copies of the buffers: ins are uploaded, outs
+// uploaded (when `keep`, for updates that read them) and downloaded.
+struct Glue {
+  const char *who;
+  hipStream_t st = nullptr;
+  std::vector<double *> dev;
+  int rc = 0;
+  explicit Glue(const char *w) : who(w)
+  {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) fail(RCG_ERR_HIP);
+  }
+  void fail(int code)
+  {
+    if (!rc) set_error(code, "%s: HIP call failed", who);
+    rc = -1;
+  }
+  double *upload(const buffer_t *b, int dims, bool copy)
+  {
+    if (rc) return nullptr;
+    const size_t bytes = (size_t)span_elems(b, dims) * sizeof(double);
+    double *p = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&p), bytes) != hipSuccess) {
+      fail(RCG_ERR_NOMEM);
+      return nullptr;
+    }
+    dev.push_back(p);
+    if (copy && hipMemcpyAsync(p, b->host, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+      fail(RCG_ERR_HIP);
+    return p;
+  }
+  void download(buffer_t *b, int dims, const double *p)
+  {
+    if (rc) return;
+    const size_t bytes = (size_t)span_elems(b, dims) * sizeof(double);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(b->host, p, bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      fail(RCG_ERR_HIP);
+  }
+  ~Glue()
+  {
+    if (st) (void)hipStreamSynchronize(st);
+    for (double *p : dev) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+unsigned blocks_for(long long n) { return (unsigned)std::min<long long>((n + 255) / 256, 65536); }
+
+int g_zero(buffer_t *b, int dims, const char *who)
+{
+  clear_error();
+  if (!host_buf(b, dims, who, "output")) return -1;
+  Glue g(who);
+  double *p = g.upload(b, dims, true);
+  const G4 o = g4_of(b, dims);
+  const long long n = region_size(o);
+  if (!g.rc) hipLaunchKernelGGL(k_g_zero, dim3(blocks_for(n)), dim3(256), 0, g.st, p, o, n);
+  g.download(b, dims, p);
+  return g.rc;
+}
+
+// rotate.cpp:33-52 on the host, in FP64
+RotPar rotation(double in_lon, double in_lat, double out_lon, double out_lat, int uvproj)
+{
+  const double pi = 4.0 * std::atan(1.0);
+  auto mul = [](const double *a, const double *b, double *c) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        c[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+  };
+  auto rx = [](double p, double *m) {
+    const double s = std::sin(p), c = std::cos(p);
+    const double r[9] = {1, 0, 0, 0, c, -s, 0, s, c};
+    for (int i = 0; i < 9; i++) m[i] = r[i];
+  };
+  auto rz = [](double p, double *m) {
+    const double s = std::sin(p), c = std::cos(p);
+    const double r[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+    for (int i = 0; i < 9; i++) m[i] = r[i];
+  };
+  double a[9], b[9], x2u[9], u2x[9], R[9];
+  rx(-(pi / 2) + out_lat, a);  // xyz2uvw(out) = rotateX(lat - pi/2) rotateZ(-pi/2 - lon)
+  rz(-(pi / 2) - out_lon, b);
+  mul(a, b, x2u);
+  rz((pi / 2) + in_lon, a);  // uvw2xyz(in) = rotateZ(pi/2 + lon) rotateX(pi/2 - lat)
+  rx((pi / 2) - in_lat, b);
+  mul(a, b, u2x);
+  mul(x2u, u2x, R);
+  RotPar rp;
+  if (uvproj != 0) {
+    // proj = (R as 2x2, inverted, third row/column identity) transposed
+    const double det = R[0] * R[4] - R[1] * R[3];
+    const double inv[9] = {R[4] / det, -R[1] / det, 0, -R[3] / det, R[0] / det, 0, 0, 0, 1};
+    const double proj[9] = {inv[0], inv[3], inv[6], inv[1], inv[4], inv[7], inv[2], inv[5], inv[8]};
+    const double Rt[9] = {R[0], R[3], R[6], R[1], R[4], R[7], R[2], R[5], R[8]};
+    double pr[9];
+    mul(proj, R, pr);
+    mul(Rt, pr, rp.m);
+  } else {
+    const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int i = 0; i < 9; i++) rp.m[i] = id[i];
+  }
+  // pos = w - R w, w = (0, 0, 1)
+  rp.pos[0] = 0.0 - R[2];
+  rp.pos[1] = 0.0 - R[5];
+  rp.pos[2] = 1.0 - R[8];
+  return rp;
+}
+
+// Halide's constraints on the 5-double visibility buffers (psf_vis.cpp:11-13,
+// 22-24; rotate.cpp:18-20, 71-73): dimension 0 is min 0, extent 5, stride 1;
+// dimension 1 has stride 5
+bool vis5(const buffer_t *b, const char *who, const char *name)
+{
+  if (!host_buf(b, 2, who, name)) return false;
+  if (b->min[0] != 0 || b->extent[0] != 5 || b->stride[0] != 1 || b->stride[1] != 5) {
+    set_error(RCG_ERR_ARG, "%s: %s must be 5 contiguous doubles per point", who, name);
+    return false;
+  }
+  return true;
+}
+
+bool covers_t(const buffer_t *in, const buffer_t *out, const char *who)
+{
+  if (in->min[1] > out->min[1] ||
+      (long long)in->min[1] + in->extent[1] < (long long)out->min[1] + out->extent[1]) {
+    set_error(RCG_ERR_ARG, "%s: the input does not cover the output's points", who);
+    return false;
+  }
+  return true;
+}
+
+int psf_vis(buffer_t *vis, buffer_t *out)
+{
+  clear_error();
+  const char *who = "kern_psf_vis";
+  if (!vis5(vis, who, "vis") || !vis5(out, who, "output") || !covers_t(vis, out, who)) return -1;
+  Glue g(who);
+  const double *dv = g.upload(vis, 2, true);
+  double *dout = g.upload(out, 2, false);
+  const G4 vg = g4_of(vis, 2), og = g4_of(out, 2);
+  const long long n = region_size(og);
+  if (!g.rc)
+    hipLaunchKernelGGL(k_g_psf_vis, dim3(blocks_for(n)), dim3(256), 0, g.st, dv, vg, dout, og, n);
+  g.download(out, 2, dout);
+  return g.rc;
+}
+
+int rotate(double in_lon, double in_lat, double out_lon0, double out_lat0, double out_lon_incr,
+           double out_lat_incr, int uvproj, buffer_t *vis, buffer_t *rot)
+{
+  clear_error();
+  const char *who = "kern_rotate";
+  if (!vis5(vis, who, "vis") || !vis5(rot, who, "output") || !covers_t(vis, rot, who)) return -1;
+  // the facet's (l, m) arrive as the output buffer's min[2], min[3]
+  const double out_lon = out_lon0 + rot->min[2] * out_lon_incr;
+  const double out_lat = out_lat0 + rot->min[3] * out_lat_incr;
+  const RotPar rp = rotation(in_lon, in_lat, out_lon, out_lat, uvproj);
+  Glue g(who);
+  const double *dv = g.upload(vis, 2, true);
+  double *dout = g.upload(rot, 2, false);
+  const G4 vg = g4_of(vis, 2), og = g4_of(rot, 2);
+  if (!g.rc)
+    hipLaunchKernelGGL(k_g_rotate, dim3(blocks_for(og.ext[1])), dim3(256), 0, g.st, dv, vg, dout,
+                       og, rp);
+  g.download(rot, 2, dout);
+  return g.rc;
+}
+
+// detile.cpp:11-12, 32: complex pairs contiguous (dimension 0 extent 2,
+// stride 1; dimension 1 stride 2)
+bool cplx3(const buffer_t *b, const char *who, const char *name)
+{
+  if (!host_buf(b, 3, who, name)) return false;
+  if (b->extent[0] != 2 || b->stride[0] != 1 || b->stride[1] != 2) {
+    set_error(RCG_ERR_ARG, "%s: %s must hold contiguous complex pairs", who, name);
+    return false;
+  }
+  return true;
+}
+
+int detile(buffer_t *in, buffer_t *out)
+{
+  clear_error();
+  const char *who = "kern_detile";
+  if (!cplx3(in, who, "in_grid") || !cplx3(out, who, "out_grid")) return -1;
+  const int x0 = std::max(out->min[1], in->min[1]);
+  const int x1 = (int)std::min((long long)out->min[1] + out->extent[1],
+                               (long long)in->min[1] + in->extent[1]);
+  const int y0 = std::max(out->min[2], in->min[2]);
+  const int y1 = (int)std::min((long long)out->min[2] + out->extent[2],
+                               (long long)in->min[2] + in->extent[2]);
+  if (x1 <= x0 || y1 <= y0) return 0;
+  Glue g(who);
+  const double *di = g.upload(in, 3, true);
+  double *dout = g.upload(out, 3, true);
+  const long long n = 2LL * (x1 - x0) * (y1 - y0);
+  if (!g.rc)
+    hipLaunchKernelGGL(k_g_detile, dim3(blocks_for(n)), dim3(256), 0, g.st, di, g4_of(in, 3),
+                       dout, g4_of(out, 3), x0, x1 - x0, y0, n);
+  g.download(out, 3, dout);
+  return g.rc;
+}
+
+// The y targets clamp(y, left, right) of y in [lo, hi] must lie in the
+// output's dimension-1 range (Halide checks the output's bounds).
+bool y_in(const buffer_t *o, long long lo, long long hi, const char *who)
+{
+  const long long left = o->min[0], right = (long long)o->min[0] + o->extent[0] - 1;
+  const long long a = std::min(std::max(lo, left), right), b = std::min(std::max(hi, left), right);
+  if (a < o->min[1] || b > (long long)o->min[1] + o->extent[1] - 1) {
+    set_error(RCG_ERR_ARG, "%s: rows %lld..%lld lie outside the output", who, a, b);
+    return false;
+  }
+  return true;
+}
+
+int defacet(buffer_t *facet, buffer_t *image)
+{
+  clear_error();
+  const char *who = "kern_defacet";
+  if (!host_buf(facet, 4, who, "facet") || !host_buf(image, 2, who, "image")) return -1;
+  const long long ylo = (long long)facet->min[1] + facet->min[3];
+  const long long yhi = ylo + facet->extent[1] - 1 + facet->extent[3] - 1;
+  if (!y_in(image, ylo, yhi, who)) return -1;
+  const int left = image->min[0], right = image->min[0] + image->extent[0] - 1;
+  Glue g(who);
+  const double *df = g.upload(facet, 4, true);
+  double *di = g.upload(image, 2, true);
+  const long long pix = (long long)image->extent[0] * image->extent[1];
+  unsigned long long *win = nullptr;
+  if (!g.rc && hipMalloc(reinterpret_cast<void **>(&win), pix * sizeof(unsigned long long)) !=
+                   hipSuccess)
+    g.fail(RCG_ERR_NOMEM);
+  const G4 fg = g4_of(facet, 4), og = g4_of(image, 2);
+  const long long n = region_size(fg);
+  if (!g.rc) {
+    hipLaunchKernelGGL(k_nv_zero, dim3(blocks_for(pix)), dim3(256), 0, g.st,
+                       reinterpret_cast<double *>(win), pix);
+    hipLaunchKernelGGL(k_g_defacet_claim, dim3(blocks_for(n)), dim3(256), 0, g.st, fg, og, left,
+                       right, win, n);
+    hipLaunchKernelGGL(k_g_defacet_write, dim3(blocks_for(n)), dim3(256), 0, g.st, df, fg, di, og,
+                       left, right, win, n);
+  }
+  g.download(image, 2, di);
+  if (win) (void)hipFree(win);
+  return g.rc;
+}
+
+int image_sum(buffer_t *image, buffer_t *sum)
+{
+  clear_error();
+  const char *who = "kern_image_sum";
+  if (!host_buf(image, 2, who, "image") || !host_buf(sum, 2, who, "sum")) return -1;
+  if (!y_in(sum, image->min[1], (long long)image->min[1] + image->extent[1] - 1, who)) return -1;
+  const int left = sum->min[0], right = sum->min[0] + sum->extent[0] - 1;
+  Glue g(who);
+  const double *di = g.upload(image, 2, true);
+  double *ds = g.upload(sum, 2, true);
+  const G4 ig = g4_of(image, 2);
+  const long long n = region_size(ig);
+  if (!g.rc)
+    hipLaunchKernelGGL(k_g_image_sum, dim3(blocks_for(n)), dim3(256), 0, g.st, di, ig, ds,
+                       g4_of(sum, 2), left, right, n);
+  g.download(sum, 2, ds);
+  return g.rc;
+}
+
 }  // namespace
 }  // namespace rcg
 
@@ -597,4 +1049,38 @@ extern "C" int kern_degrid_gpu1(const double scale, const int32_t grid_size,
                                 buffer_t *vis, buffer_t *vis_out)
 {
   return rcg::degrid(scale, grid_size, margin_size, gcf, uvg, vis, vis_out, true, -333);
+}
+
+// MS6 imaging glue (Halide generators in MS6/kernel/cpu/gridding)
+extern "C" int kern_init(buffer_t *grid) { return rcg::g_zero(grid, 3, "kern_init"); }
+
+extern "C" int kern_image_init(buffer_t *image)
+{
+  return rcg::g_zero(image, 2, "kern_image_init");
+}
+
+extern "C" int kern_psf_vis(buffer_t *vis, buffer_t *out) { return rcg::psf_vis(vis, out); }
+
+extern "C" int kern_rotate(const double in_lon, const double in_lat, const double out_lon0,
+                           const double out_lat0, const double out_lon_incr,
+                           const double out_lat_incr, const int32_t uvproj, buffer_t *vis,
+                           buffer_t *rot)
+{
+  return rcg::rotate(in_lon, in_lat, out_lon0, out_lat0, out_lon_incr, out_lat_incr, uvproj, vis,
+                     rot);
+}
+
+extern "C" int kern_detile(buffer_t *in_grid, buffer_t *out_grid)
+{
+  return rcg::detile(in_grid, out_grid);
+}
+
+extern "C" int kern_defacet(buffer_t *facet, buffer_t *image)
+{
+  return rcg::defacet(facet, image);
+}
+
+extern "C" int kern_image_sum(buffer_t *image, buffer_t *sum)
+{
+  return rcg::image_sum(image, sum);
 }

@@ -1,6 +1,7 @@
 """ctypes view of the MS6 Halide buffer_t ABI (include/rcgrid_halide.h) and
 calls of kern_scatter / kern_degrid / kern_scatter_gpu1 / kern_degrid_gpu1 /
-nvGridder.
+nvGridder, and of the imaging-glue kernels kern_init / kern_image_init /
+kern_psf_vis / kern_rotate / kern_detile / kern_defacet / kern_image_sum.
 
 Array conventions (the MS6 generators' buffer layouts):
   vis  float64 [n, 5]      (u, v, w, re, im)
@@ -120,3 +121,47 @@ def grid_gpu(scale: float, gcf: np.ndarray, img_dim: int, combined: np.ndarray |
     if code:
         raise _lib.RcgError(f"gridGPU failed: {msg}")
     return out
+
+
+def buffer_of(arr: np.ndarray, mins=(0, 0, 0, 0)) -> BufferT:
+    """A buffer_t over a float64 array whose last index is Halide dimension 0
+    (shape [..., x1, x0]); mins in Halide order (x0 first)."""
+    assert arr.dtype == np.float64 and arr.ndim <= 4
+    ext = arr.shape[::-1]
+    st = tuple(s // 8 for s in arr.strides[::-1])
+    return _buf(arr, ext, st, tuple(mins) + (0,) * (4 - len(mins)))
+
+
+def _call(name: str, *args) -> int:
+    conv = [C.addressof(a) if isinstance(a, BufferT) else a for a in args]
+    return int(getattr(_lib.lib(), name)(*conv))
+
+
+def init(b: BufferT) -> int:
+    return _call("kern_init", b)
+
+
+def image_init(b: BufferT) -> int:
+    return _call("kern_image_init", b)
+
+
+def psf_vis(vis_b: BufferT, out_b: BufferT) -> int:
+    return _call("kern_psf_vis", vis_b, out_b)
+
+
+def rotate(in_lon, in_lat, out_lon0, out_lat0, lon_incr, lat_incr, uvproj, vis_b: BufferT,
+           out_b: BufferT) -> int:
+    return _call("kern_rotate", in_lon, in_lat, out_lon0, out_lat0, lon_incr, lat_incr,
+                 int(uvproj), vis_b, out_b)
+
+
+def detile(in_b: BufferT, out_b: BufferT) -> int:
+    return _call("kern_detile", in_b, out_b)
+
+
+def defacet(facet_b: BufferT, image_b: BufferT) -> int:
+    return _call("kern_defacet", facet_b, image_b)
+
+
+def image_sum(image_b: BufferT, sum_b: BufferT) -> int:
+    return _call("kern_image_sum", image_b, sum_b)

@@ -159,3 +159,59 @@ def test_bin_sort_and_raw_gridders(hip_device):
     pos = np.ascontiguousarray(vis[:, :3])
     vals = np.ascontiguousarray(vis[:, 3:])
     assert close(HB.grid_gpu(1.0, gcf, 128, pos=pos, vals=vals), exp)
+
+
+# --- MS6 imaging glue --------------------------------------------------------
+
+def test_init_and_psf_vis(hip_device):
+    g = np.ones((6, 5, 2))
+    assert HB.init(HB.buffer_of(g, (0, 3, 4))) == 0 and not g.any()
+    im = np.ones((7, 9))
+    assert HB.image_init(HB.buffer_of(im, (2, 2))) == 0 and not im.any()
+    rng = np.random.default_rng(0)
+    vis = rng.standard_normal((50, 5))
+    out = np.zeros((20, 5))
+    # output points 10..29 of an input holding points 5..54
+    assert HB.psf_vis(HB.buffer_of(vis, (0, 5)), HB.buffer_of(out, (0, 10))) == 0
+    assert np.array_equal(out, H.psf_vis(vis[5:25]))
+    # an output that reaches past the input is refused
+    assert HB.psf_vis(HB.buffer_of(vis, (0, 40)), HB.buffer_of(out, (0, 30))) == -1
+
+
+@pytest.mark.parametrize("uvproj", [0, 1])
+def test_rotate_vs_oracle(hip_device, uvproj):
+    rng = np.random.default_rng(5 + uvproj)
+    vis = np.column_stack([rng.uniform(-3e3, 3e3, (400, 3)), rng.standard_normal((400, 2))])
+    vis = np.ascontiguousarray(vis)
+    out = np.zeros_like(vis)
+    args = (0.4, -0.8, 0.39, -0.805, 1e-3, 2e-3, uvproj)
+    ob = HB.buffer_of(out)
+    ob.min[2], ob.min[3] = 3, -2          # the facet (l, m)
+    assert HB.rotate(*args, HB.buffer_of(vis), ob) == 0
+    exp = H.rotate(vis, *args, 3, -2)
+    assert close(out, exp)
+
+
+def test_detile_defacet_image_sum_vs_oracle(hip_device):
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((30, 40, 2))
+    out = rng.standard_normal((50, 35, 2))
+    exp = out.copy()
+    H.detile(a, (10, 5), exp, (25, 0))
+    assert HB.detile(HB.buffer_of(a, (0, 10, 5)), HB.buffer_of(out, (0, 25, 0))) == 0
+    assert close(out, exp)
+    # facets at l, m in 0..2 shifted by x + l, y + m: neighbours overlap and
+    # the right and bottom edges clamp onto the border, so many pixels have
+    # several writers and the last one in Halide order must win
+    facet = rng.standard_normal((3, 3, 16, 16))       # [m][l][y][x]
+    img = rng.standard_normal((17, 17))
+    expi = img.copy()
+    H.defacet(facet, (0, 0, 0, 0), expi, (0, 0))
+    assert HB.defacet(HB.buffer_of(facet), HB.buffer_of(img)) == 0
+    assert np.array_equal(img, expi)
+    src = rng.standard_normal((20, 20))
+    s = np.zeros((10, 10))
+    exps = s.copy()
+    H.image_sum(src, (-3, -3), exps, (2, 2))
+    assert HB.image_sum(HB.buffer_of(src, (-3, -3)), HB.buffer_of(s, (2, 2))) == 0
+    assert close(s, exps)

@@ -180,3 +180,77 @@ def test_nv_grid_matches_the_reference_gather_loops():
                                                             16 * b + a + 119]
     got = H.nv_grid(vis, gcf, 1.0, N)
     assert np.allclose(got[..., 0] + 1j * got[..., 1], exp, rtol=0, atol=1e-12)
+
+
+# --- MS6 imaging glue -------------------------------------------------------
+
+def test_rotation_at_the_pole_is_a_z_rotation():
+    # uvw2xyz(lon, pi/2) = Z(pi/2 + lon), xyz2uvw(lon', pi/2) = Z(-pi/2 - lon'),
+    # so R = Z(lon - lon'): no phase, uv turned by lon - lon' under uvproj
+    pi = np.pi
+    R, mtx, pos = H.rotation(0.3, pi / 2, 0.1, pi / 2, 1)
+    d = 0.3 - 0.1
+    Z = np.array([[np.cos(d), -np.sin(d), 0], [np.sin(d), np.cos(d), 0], [0, 0, 1]])
+    assert np.allclose(R, Z, atol=1e-15) and np.allclose(mtx, Z, atol=1e-15)
+    assert np.allclose(pos, 0, atol=1e-15)
+    _, ident, _ = H.rotation(0.3, pi / 2, 0.1, pi / 2, 0)
+    assert np.array_equal(ident, np.eye(3))
+
+
+def test_rotation_in_latitude_gives_the_phase_path():
+    # at lon 0: R = X(lat' - pi/2) X(pi/2 - lat) = X(delta), pos = w - X(delta) w
+    # = (0, sin delta, 1 - cos delta)
+    lat, delta = -0.7, 0.02
+    R, _, pos = H.rotation(0.0, lat, 0.0, lat + delta, 0)
+    X = np.array([[1, 0, 0], [0, np.cos(delta), -np.sin(delta)], [0, np.sin(delta), np.cos(delta)]])
+    assert np.allclose(R, X, atol=1e-15)
+    assert np.allclose(pos, [0, np.sin(delta), 1 - np.cos(delta)], atol=1e-15)
+    vis = np.array([[10.0, 20.0, 5.0, 1.0, 0.0]])
+    out = H.rotate(vis, 0.0, lat, 0.0, lat, 0.0, delta, 0, 0, 1)  # facet m = 1
+    phi = 2 * np.pi * (20.0 * np.sin(delta) + 5.0 * (1 - np.cos(delta)))
+    assert np.allclose(out[0], [10.0, 20.0, 5.0, np.cos(phi), np.sin(phi)], atol=1e-12)
+
+
+def test_rotation_is_orthogonal():
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        a = rng.uniform(-1.5, 1.5, 4)
+        R, mtx, _ = H.rotation(*a, 1)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(R) - 1) < 1e-14
+        # the projected transform keeps the w axis of the rotated frame
+        assert np.isfinite(mtx).all()
+
+
+def test_detile_adds_the_overlap():
+    a = np.arange(3 * 4 * 2, dtype=float).reshape(3, 4, 2)   # x 5..8, y 2..4
+    out = np.ones((4, 3, 2))                                  # x 7..9, y 3..6
+    H.detile(a, (5, 2), out, (7, 3))
+    exp = np.ones((4, 3, 2))
+    # overlap x 7..8, y 3..4
+    exp[0:2, 0:2] += a[1:3, 2:4]
+    assert np.array_equal(out, exp)
+
+
+def test_defacet_last_write_wins_and_clamps_rows_with_the_x_bounds():
+    # image x 2..9, y 0..11; facets of 2x2 at l in 0..1, m = 0, x, y from 8
+    img = np.full((12, 8), -1.0)
+    facet = np.arange(1 * 2 * 2 * 2, dtype=float).reshape(1, 2, 2, 2) + 1  # [m][l][y][x]
+    H.defacet(facet, (8, 8, 0, 0), img, (2, 0))
+    # x + l: 8, 9, 9, 10 -> clamp to 2..9: 8, 9, 9, 9; y + m: 8, 9 (inside 2..9)
+    # the last writer of (9, y) is l = 1, x = 9
+    assert img[8, 8 - 2] == facet[0, 0, 0, 0] and img[8, 9 - 2] == facet[0, 1, 0, 1]
+    assert img[9, 9 - 2] == facet[0, 1, 1, 1]
+    # rows outside 2..9 are never written, whatever y + m is
+    facet2 = np.ones((1, 1, 3, 1))
+    img2 = np.zeros((12, 8))
+    H.defacet(facet2, (2, 9, 0, 1), img2, (2, 0))   # y + m = 10, 11, 12 -> row 9
+    assert img2[9, 0] == 1.0 and np.count_nonzero(img2) == 1
+
+
+def test_image_sum_accumulates_clamped_borders():
+    img = np.ones((4, 4))                # x 0..3, y 0..3
+    out = np.zeros((2, 2))               # x 1..2, y 1..2
+    H.image_sum(img, (0, 0), out, (1, 1))
+    # corner (1, 1) takes x in 0..1, y in 0..1: 4 cells; likewise every corner
+    assert np.array_equal(out, np.full((2, 2), 4.0))
