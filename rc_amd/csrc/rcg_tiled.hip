@@ -684,14 +684,16 @@ __global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restri
 // lanes accumulate four partial products, which are summed across the
 // wavefront four pairs at a time.
 
-// Sums of 4 pairs at once.  When a pair ends, every lane leaves its part of
-// the pair's sum in the wavefront's LDS buffer, row slot*64 + lane, and lane
-// 0 the output address in meta[slot].  After 4 pairs (or at the end of an
-// item) lane l sums entries c, c + 16, c + 32, c + 48 of pair l / 16 (c =
-// l % 16), then a DPP prefix sum over its row of 16 lanes (row_shr 1, 2,
-// 4, 8 with zero fill) leaves the pair's total in the row's last lane,
-// which stores it.  The order of the additions is fixed, so the result is
-// reproducible.
+// Sums of 4 pairs at once.  Pair k of a wavefront (in the order it walks
+// them) has its output address written to meta[k % 8] by lane 0 when the
+// pair is decoded (two halves of 4, so the next pair's address never
+// overwrites one still to be used), and when it ends every lane leaves its
+// part of the pair's sum in row k % 4 of the wavefront's LDS buffer.  After
+// pairs 4a..4a+3 (or at the end of an item) lane l sums entries c, c + 16,
+// c + 32, c + 48 of row l / 16 (c = l % 16), then a DPP prefix sum over its
+// row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill) leaves the pair's
+// total in the row's last lane, which stores it.  The order of the
+// additions is fixed, so the result is reproducible.
 constexpr int SUMS = 4;
 // Output addresses kept in LDS are typed global, so the final store is a
 // global (not flat) store and does not hold up the wavefront's LDS waits;
@@ -740,7 +742,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
   constexpr int DUMMY = T * LS;
   __shared__ float2 tile[T * LS + 2];
   __shared__ float2 sums[4][SUMS * 64];
-  __shared__ gu64 *meta[4][SUMS];
+  __shared__ gu64 *meta[4][2 * SUMS];
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -765,12 +767,16 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n);
       int k = 0, s0 = 0;
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
+      auto out_addr = [&](const Pair &q) {
+        return (gu64 *)(q.whole ? vis + q.point : d.partial + (long long)q.point * d.mpp + q.slot);
+      };
+      if (lane == 0) my_meta[0] = out_addr(p);
+      float sg = p.conj ? -1.f : 1.f;
       float2 ta[GRP], tb[GRP];
       int la[GRP], lb[GRP];
       load_group(ta, la, p, 0, DUMMY * 8, rsrc);
       // P = sum gx*tx, Q = sum gy*ty, R = sum gx*ty, U = sum gy*tx
       float P = 0.f, Q = 0.f, R = 0.f, U = 0.f;
-      int ns = 0;  // pairs in the sum buffer
       bool more = true;
       // one group; alternate buffers as in the gridder
       auto body = [&](float2 (&tc)[GRP], int (&lc)[GRP], float2 (&tn)[GRP], int (&ln)[GRP]) {
@@ -780,6 +786,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
 #pragma unroll
         for (int u = 0; u < GRP; u++) g[u] = lds_at(tile, lc[u]);
         Pair pn = p;
+        float sgn = sg;
         int kn = k, sn = s0 + GRP;
         const bool last = sn >= p.nslabs;
         if (last) {
@@ -793,8 +800,13 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  0);
+            if (lane == 0) my_meta[kn & (2 * SUMS - 1)] = out_addr(pn);
+            sgn = pn.conj ? -1.f : 1.f;
           } else {
-            pn.h = 0;  // past the end: the empty pair, no lane active
+            // past the end: the empty pair, no lane active; it never ends,
+            // so no sum is written for it
+            pn.h = 0;
+            pn.nslabs = 1 << 20;
           }
         }
         more = kn < npw;
@@ -807,23 +819,18 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
           R = fmaf(g[u].x, tc[u].y, R);
           U = fmaf(g[u].y, tc[u].x, U);
         }
-        if (last && k < npw) {
+        if (last) {
           // tap t' = (tx, sg*ty): re = P - sg*Q, im = sg*R + U (sg = +-1, so
-          // each fma rounds as the add or subtract would)
-          const float sg = p.conj ? -1.f : 1.f;
-          my_sums[ns * 64 + lane] = make_float2(fmaf(-sg, Q, P), fmaf(sg, R, U));
-          // a point inside one tile is final; a split point leaves its part
-          // in its slot for k_degrid_reduce
-          if (lane == 0)
-            my_meta[ns] = (gu64 *)(p.whole ? vis + p.point
-                                           : d.partial + (long long)p.point * d.mpp + p.slot);
-          if (++ns == SUMS) {
-            flush_sums(my_sums, my_meta, SUMS, lane);
-            ns = 0;
-          }
+          // each fma rounds as the add or subtract would); a point inside
+          // one tile is final, a split point leaves its part in its slot for
+          // k_degrid_reduce
+          my_sums[(k & (SUMS - 1)) * 64 + lane] = make_float2(fmaf(-sg, Q, P), fmaf(sg, R, U));
+          if ((k & (SUMS - 1)) == SUMS - 1)
+            flush_sums(my_sums, my_meta + (k & SUMS), SUMS, lane);
           P = Q = R = U = 0.f;
         }
         p = pn;
+        sg = sgn;
         k = kn;
         s0 = sn;
       };
@@ -835,7 +842,8 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
         body(ta, la, tb, lb);
         body(tb, lb, ta, la);
       } while (more);
-      if (ns > 0) flush_sums(my_sums, my_meta, ns, lane);
+      if (npw & (SUMS - 1))
+        flush_sums(my_sums, my_meta + ((npw - 1) & SUMS), npw & (SUMS - 1), lane);
       // never true (chunk >= 1): keeps the prefetch loads from sinking
       if (d.chunk < 0) {
         float2 z = make_float2(0.f, 0.f);
