@@ -20,7 +20,7 @@
 //     and steal from the other queues when theirs is empty;
 //  4. gridding: each wavefront accumulates its pairs' taps into a private
 //     FP32 copy of the tile in LDS, lanes covering the rows and columns of
-//     the window/tile intersection, GCF rows read coalesced; the four copies
+//     the window/tile intersection, GCF rows read coalesced; the copies
 //     are summed and stored to HBM with plain stores when the tile has a
 //     single item, with FP32 atomics (non-zero cells only) otherwise;
 //  5. degridding: the workgroup stages its grid tile in LDS, each wavefront
@@ -424,6 +424,18 @@ __device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned vo
 // 3-6% slower, 4 slabs 10-15%, 1 slab 13-17%).
 constexpr int GRP = 2;
 
+// Wavefronts per gridding workgroup.  Each holds a private copy of the tile
+// in LDS, so the copies' size sets how many workgroups share a CU's 160 KB:
+// at T = 32 three wavefronts (25 KB) let six workgroups (18 wavefronts) in,
+// against four of four (16); measured 2-3% faster on the 4096^2 w-projection
+// workload (one and two wavefronts: equal or slower).  At T = 64 three
+// copies would leave one 3-wavefront workgroup per CU.
+template <int T>
+constexpr int grid_waves()
+{
+  return T == 32 ? 3 : 4;
+}
+
 struct Pair {
   int h, rows, nslabs, point, whole, conj, slot;
   int lr;          // lane row (idle lanes: 64)
@@ -483,20 +495,21 @@ __device__ __forceinline__ float2 &lds_at(float2 *lds, int byte)
   return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + byte);
 }
 
-// A wavefront's pairs are ir.begin + wave + 4k, each cut into groups of GRP
+// A wavefront's pairs are ir.begin + wave + nw k (nw wavefronts in the
+// workgroup), each cut into groups of GRP
 // slabs.  Their records (and visibilities) come 64 at a time, one per lane,
 // double-buffered a batch ahead and broadcast with readlane; batch b is
 // loaded by stream_batch.
 template <bool VIS>
 __device__ __forceinline__ void stream_batch(const TiledDev &d, const float2 *__restrict__ vis,
                                              const ItemRange &ir, int wave, int lane, int npw,
-                                             int b, int4 &rec, float2 &v)
+                                             int b, int4 &rec, float2 &v, int nw)
 {
   const int k = b * 64 + lane;
   rec = make_int4(0, 0, 0, 0);
   v = make_float2(0.f, 0.f);
   if (k < npw) {
-    rec = d.pairs[ir.begin + wave + 4 * k];
+    rec = d.pairs[ir.begin + wave + nw * k];
     if (VIS) v = vis[rec.w];
   }
 }
@@ -517,32 +530,33 @@ __device__ __forceinline__ float two_sum_err(float a, float b)
 // read-modify-write of a cell needs no atomic: within one pair every lane
 // and slab has its own cell (lanes outside the intersection only ever add
 // zero to the dummy cell), and a wavefront's LDS operations complete in
-// program order; the four copies are summed at the flush.
+// program order; the copies are summed at the flush.
 
 template <int T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * grid_waves<T>())
 k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
              const float2 *__restrict__ taps, long long n_taps,
              float2 *__restrict__ grid)
 {
+  constexpr int GW = grid_waves<T>();
   constexpr int LS = T + 1;
   constexpr int CP = T * LS + 2;  // one copy (+ dummy cell)
   constexpr int DUMMY = T * LS;
-  __shared__ float2 acc[4 * CP];
+  __shared__ float2 acc[GW * CP];
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // LDS byte addresses of the wavefront's copy and of its dummy cell
   const int base = wave * CP * 8, dummy = (wave * CP + DUMMY) * 8;
   for_each_item(d, d.queue, [&](const ItemRange &ir) {
-    for (int q = threadIdx.x; q < 4 * CP; q += 256) acc[q] = make_float2(0.f, 0.f);
+    for (int q = threadIdx.x; q < GW * CP; q += 64 * GW) acc[q] = make_float2(0.f, 0.f);
     __syncthreads();
-    const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
+    const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + GW - 1) / GW : 0;
     if (npw > 0) {
       int4 rec, rec_n;
       float2 mv, mv_n;
-      stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv);
-      stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n);
+      stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv, GW);
+      stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n, GW);
       int k = 0, s0 = 0;
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane,
                                base);
@@ -575,7 +589,7 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
             if (j == 0) {
               rec = rec_n;
               mv = mv_n;
-              stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n);
+              stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n, GW);
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  base);
@@ -630,17 +644,20 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       }
     }
     __syncthreads();
-    // flush: sum the four copies
+    // flush: sum the copies in order
     const int gsz = d.grid_size;
     float2 *gi = grid + ir.goff;
-    for (int q = threadIdx.x; q < T * T; q += 256) {
+    for (int q = threadIdx.x; q < T * T; q += 64 * GW) {
       const int r = q / T, cc = q - (q / T) * T;
       const int gu = ir.tu0 + r, gv = ir.tv0 + cc;
       if (gu >= gsz || gv >= gsz) continue;
       const int l = r * LS + cc;
-      const float2 s0 = acc[l], s1 = acc[CP + l], s2 = acc[2 * CP + l], s3 = acc[3 * CP + l];
-      const float re = (s0.x + s1.x) + (s2.x + s3.x);
-      const float im = (s0.y + s1.y) + (s2.y + s3.y);
+      float re = acc[l].x, im = acc[l].y;
+#pragma unroll
+      for (int c = 1; c < GW; c++) {
+        re += acc[c * CP + l].x;
+        im += acc[c * CP + l].y;
+      }
       float2 *g = gi + (long long)gu * d.pitch + gv;
       if (ir.single) {
         *g = make_float2(re, im);
@@ -763,8 +780,8 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
     if (npw > 0) {
       int4 rec, rec_n;
       float2 dm, dm_n;
-      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm);
-      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n);
+      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm, 4);
+      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n, 4);
       int k = 0, s0 = 0;
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
       auto out_addr = [&](const Pair &q) {
@@ -796,7 +813,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
             const int j = kn & 63;
             if (j == 0) {
               rec = rec_n;
-              stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n);
+              stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n, 4);
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  0);
@@ -902,12 +919,12 @@ hipError_t dalloc(T **p, size_t n)
 
 // Persistent workgroups of a kernel: resident workgroups per CU x CUs,
 // a multiple of 8 (one share per XCD).
-hipError_t persistent_blocks(const void *kernel, int *nb)
+hipError_t persistent_blocks(const void *kernel, int block, int *nb)
 {
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
   if (e != hipSuccess) return e;
   *nb = std::max(8, (std::max(1, per_cu) * std::max(1, cus)) & ~7);
   return hipSuccess;
@@ -917,9 +934,10 @@ template <int T>
 hipError_t tile_kernels(int *grid_blocks, int *degrid_blocks)
 {
   hipError_t e = persistent_blocks(reinterpret_cast<const void *>(&k_grid_tiles<T>),
-                                   grid_blocks);
+                                   64 * grid_waves<T>(), grid_blocks);
   if (e == hipSuccess)
-    e = persistent_blocks(reinterpret_cast<const void *>(&k_degrid_tiles<T>), degrid_blocks);
+    e = persistent_blocks(reinterpret_cast<const void *>(&k_degrid_tiles<T>), 256,
+                          degrid_blocks);
   return e;
 }
 
@@ -1184,9 +1202,15 @@ rcg_status grid_tiled(rcg_tiled_plan *pl, const rcg_gcf *gcf, const rcg_c64 *vis
   hipEvent_t *ev = timing_slot(pl, 0);
   if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
   switch (d.T) {
-    case 16: hipLaunchKernelGGL(k_grid_tiles<16>, nb, dim3(256), 0, st, d, v, t, nt, g); break;
-    case 32: hipLaunchKernelGGL(k_grid_tiles<32>, nb, dim3(256), 0, st, d, v, t, nt, g); break;
-    default: hipLaunchKernelGGL(k_grid_tiles<64>, nb, dim3(256), 0, st, d, v, t, nt, g); break;
+    case 16:
+      hipLaunchKernelGGL(k_grid_tiles<16>, nb, dim3(64 * grid_waves<16>()), 0, st, d, v, t, nt, g);
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_grid_tiles<32>, nb, dim3(64 * grid_waves<32>()), 0, st, d, v, t, nt, g);
+      break;
+    default:
+      hipLaunchKernelGGL(k_grid_tiles<64>, nb, dim3(64 * grid_waves<64>()), 0, st, d, v, t, nt, g);
+      break;
   }
   RCG_HIP_TRY(hipGetLastError());
   if (ev) {
