@@ -29,6 +29,7 @@ struct TiledDev {
   int ppt;           // points per binning thread
   long long gstride; // grid elements per instance (grid_size * pitch)
   long long pair_cap;
+  long long npts;    // points of the plan (all instances)
   int *tile_count;   // pairs per tile
   int *tile_start;   // first pair of each tile
   int *tile_cursor;  // fill cursors
@@ -39,7 +40,7 @@ struct TiledDev {
   const int *pt_off;   // n_inst + 1: first point of each instance
   const int *blk_off;  // n_inst + 1: first binning workgroup of each instance
   int4 *pairs;       // (point, tile) records sorted by tile
-  float2 *partial;   // degridding: per (point, tile slot) partial sums
+  float2 *partial;   // degridding: partial sums, [tile slot][point]
   int adj;           // gridding with conjugated taps (set per launch)
   float2 *lo;        // compensated flush: rounding errors of the atomic adds
 };

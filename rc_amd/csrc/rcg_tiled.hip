@@ -785,7 +785,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       int k = 0, s0 = 0;
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
       auto out_addr = [&](const Pair &q) {
-        return (gu64 *)(q.whole ? vis + q.point : d.partial + (long long)q.point * d.mpp + q.slot);
+        return (gu64 *)(q.whole ? vis + q.point : d.partial + (long long)q.slot * d.npts + q.point);
       };
       if (lane == 0) my_meta[0] = out_addr(p);
       float sg = p.conj ? -1.f : 1.f;
@@ -890,11 +890,14 @@ k_degrid_reduce(TiledDev d, const rcg_pg *__restrict__ pg, long long n,
   point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
   const int np = (tu1 - tu0 + 1) * (tv1 - tv0 + 1);
   if (np <= 1 || np > d.mpp) return;  // whole (written by the tile kernel)
-  const float2 *q = d.partial + i * d.mpp;
+  // partials are stored slot-major, so consecutive points read consecutive
+  // elements of each slot
+  const float2 *q = d.partial + i;
   float2 s = q[0];
   for (int k = 1; k < np; k++) {
-    s.x += q[k].x;
-    s.y += q[k].y;
+    const float2 t = q[(long long)k * d.npts];
+    s.x += t.x;
+    s.y += t.y;
   }
   vis[i] = s;
 }
@@ -1065,6 +1068,7 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
   d.ppt = ppt;
   d.gstride = (long long)size * pitch;
   d.pair_cap = n * max_pairs_pt;
+  d.npts = n;
   const int n_seg = ceil_div(d.n_tiles, SCAN_SEG);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bin_fill),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
