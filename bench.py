@@ -43,14 +43,15 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision"],
+    ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision", "binner"],
                     default="wproj",
                     help="wproj: the 4096^2 w-projection workload (default); batched: 64 "
                          "uneven 2048^2 snapshot instances, one graph launch vs one launch "
                          "per instance; imaging: the differentiable measurement operator "
                          "on a 2048^2 sky, 20 gradient steps and 1000 Hogbom iterations; "
                          "precision: 50M points of the w-projection layout, FP32 with and "
-                         "without compensated flushes against the FP64 path")
+                         "without compensated flushes against the FP64 path; binner: the "
+                         "MS4 uv binner (binm) through its host-memory C ABI")
     ap.add_argument("--instances", type=int, default=64)
     return ap.parse_args()
 
@@ -395,6 +396,69 @@ def bench_precision(args) -> int:
     return 0
 
 
+def bench_binner(args) -> int:
+    """The MS4 uv binner (binm: 32 x 32 bins, margin copies, half GCF
+    indices) through its host-memory C ABI on the points of the w-projection
+    workload, moved to non-negative cells (the reference bins the
+    untranslated cell, MS4/kernel/cpu/binner/binner.cpp:133-139).  One step =
+    one binm call: upload of uvw and amplitudes, the device passes, and the
+    copy of the binned block into pinned host memory.  value = points per
+    second through the call (PCIe included).  The CPU baseline is the
+    oracle's single-threaded restatement on the same points."""
+    import ctypes as C
+    from rc_amd import _lib, synth
+
+    vc, gcf, N = synth.wproj(n_times=args.n_times)
+    uvw = np.ascontiguousarray(vc.uvw.reshape(-1, 3), np.float64)
+    uvw[:, :2] += (N / 2) / vc.scale
+    vis = np.ascontiguousarray(vc.vis.reshape(-1), np.complex128)
+    sup = np.ascontiguousarray(gcf.supps, np.int32)
+    M = gcf.max_w_plane
+    n = len(vis)
+    h = _lib.lib()
+    centre = sup.ctypes.data + 4 * M
+
+    def call():
+        p = h.binm(n, vc.scale, vc.wstep, vis.ctypes.data, uvw.ctypes.data, centre, N)
+        if not p:
+            raise _lib.RcgError(f"binm: {_lib.last_error()[1]}")
+        size = C.c_int.from_address(p - 12320).value
+        h.rcg_bin_free(p)
+        return size
+
+    for _ in range(args.warmup):
+        size = call()
+    ts = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        call()
+        ts.append(time.perf_counter() - t)
+    ms = float(np.median(ts)) * 1e3
+    copies = (size - 12320) // 24
+    line = {
+        "metric": "Mpoints/s binned by binm (32x32 uv bins with margin copies), host memory in "
+                  "and out",
+        "value": round(n / (ms / 1e3) / 1e6, 3), "unit": "Mpoints/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (int16 records)", "data": "synthetic",
+        "config": {"workload": "w-projection points (256 antennas x 306 times) on 4096x4096, "
+                               "32 w-planes, supports 8-64, cells moved to [0, 4096)",
+                   "points": n, "copies": int(copies), "block_bytes": int(size)},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        O.bin_points(uvw, vis, vc.scale, vc.wstep, sup, M, N, True)
+        cpu_s = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": round(n / cpu_s / 1e6, 3), "unit": "Mpoints/s",
+                                "cores": 1, "kind": "port",
+                                "sample": f"all {n} points, oracle orc_bin (C, one thread), "
+                                          f"{cpu_s:.2f} s including the copy out"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -415,6 +479,8 @@ def main() -> int:
         return bench_imaging(args)
     if args.workload == "precision":
         return bench_precision(args)
+    if args.workload == "binner":
+        return bench_binner(args)
     dev = torch.device("cuda:0")
     t0 = time.time()
     vc, gcf, N = synth.wproj(n_times=args.n_times)

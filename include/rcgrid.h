@@ -147,6 +147,42 @@ void rcg_gcf_pad_quirk(int enable);
 void calcAccums(const rcg_double3 uvw[], double sums[], int npts[],
                 double wstep, int numDataPoints, int numOfWPlanes);
 
+/* Replace bin / binm, MS4/kernel/common/binner.h:8-9 (MS4/kernel/cpu/binner/
+ * binner.cpp:100-189; called from MS4/dna-programs/Kernel/GPU/
+ * GatherGrid.hs:20-44).  Bins num_points points into 32 x 32 uv bins of
+ * grid_size/32 cells: bin (p.u / ds, p.v / ds) of the untranslated pregrid
+ * cell (C truncating division), plus copies in the neighbour bins that the
+ * margin supp/2 reaches, supp = gcfsupps[w_plane] (centred table).  Each copy
+ * carries the 8-byte Pregridded record (u, v translated by
+ * grid_size/2 - supp/2, layer index, supp) and the amplitude times
+ * exp(2 pi i w scale); binm negates the layer index for w < 0.  Each bin
+ * keeps its copies in input order.  The result is one pinned host block laid
+ * out as the reference's: an rcg_bin_table (RCG_BIN_TABLE_BYTES, padded),
+ * then every bin's amplitudes, then every bin's records; offsets count bytes
+ * from the end of the table.  Returns the address past the table, as the
+ * reference does (its table is at ret - RCG_BIN_TABLE_BYTES).  Free with
+ * rcg_bin_free(ret).  Returns NULL with rcg_last_error() set where the
+ * reference fails its assert (a home bin outside [0, 32)^2), where a field
+ * leaves int16 or the block exceeds 2^31 - 1 bytes.  Empty bins get
+ * n_items = 0 and the offset where they would start (unset in the
+ * reference). */
+#define RCG_BIN_DIVS 32
+#define RCG_BIN_TABLE_BYTES 12320
+typedef struct rcg_bin_table {
+  int data_size;                               /* whole block, bytes     */
+  int data_off;                                /* RCG_BIN_TABLE_BYTES    */
+  int vis_off[RCG_BIN_DIVS][RCG_BIN_DIVS];
+  int pre_off[RCG_BIN_DIVS][RCG_BIN_DIVS];
+  int n_items[RCG_BIN_DIVS][RCG_BIN_DIVS];
+} rcg_bin_table;
+void *bin(int num_points, double scale, double wstep, rcg_c128 *amps,
+          rcg_double3 *uvws, int gcfsupps[], int grid_size);
+void *binm(int num_points, double scale, double wstep, rcg_c128 *amps,
+           rcg_double3 *uvws, int gcfsupps[], int grid_size);
+/* Releases a block returned by bin / binm (the reference has no release
+ * function; its block comes from cudaHostAlloc). */
+void rcg_bin_free(void *data);
+
 /* Thread-local error of the last call made on this thread (0 = none). */
 int rcg_last_error(void);
 const char *rcg_last_error_string(void);

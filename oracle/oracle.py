@@ -160,3 +160,30 @@ def pregrid(vc: VisChunks, grid_size: int, half: bool) -> np.ndarray:
 
 def default_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def bin_points(uvw: np.ndarray, vis: np.ndarray, scale: float, wstep: float,
+               gcf_supps: np.ndarray, max_w_plane: int, grid_size: int,
+               mirror: bool = False):
+    """orc_bin (the MS4 binner restated): returns (counts[32, 32], pre
+    (total, 4) int16, vis (total,) complex128, src (total,) point index) in
+    bin order, or None where the reference would fail its assert."""
+    h = oracle_lib()
+    h.orc_bin.restype = C.c_longlong
+    h.orc_bin.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, vp, vp, vp, C.c_int,
+                          vp, vp, vp, vp, C.c_longlong]
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex128)
+    sup = np.ascontiguousarray(gcf_supps, np.int32)
+    centre = sup.ctypes.data + 4 * max_w_plane
+    counts = np.zeros(32 * 32, np.int32)
+    args = (int(mirror), len(vis), scale, wstep, vis.ctypes.data, uvw.ctypes.data, centre,
+            grid_size, counts.ctypes.data)
+    total = h.orc_bin(*args, None, None, None, 0)
+    if total < 0:
+        return None
+    pre = np.zeros((total, 4), np.int16)
+    out = np.zeros(total, np.complex128)
+    src = np.zeros(total, np.int32)
+    h.orc_bin(*args, pre.ctypes.data, out.ctypes.data, src.ctypes.data, total)
+    return counts.reshape(32, 32), pre, out, src

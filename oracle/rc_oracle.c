@@ -27,6 +27,19 @@
  *   orc_herm_padded, orc_fftshift_even
  *                        closed forms (index mirror, quadrant swap is an
  *                        involution equal to numpy.fft.fftshift for even N)
+ *   orc_bin              the MS4 binner: its pregrid part is orc_pregrid
+ *                        (pinned above) minus the translation; bins, margin
+ *                        copies and order are pinned by a geometric check
+ *                        independent of the if / else-if form (a copy goes
+ *                        to each neighbour bin that the window
+ *                        [p.u - m, p.u + m) x [p.v - m, p.v + m) reaches,
+ *                        m <= size/64 so at most one side per axis), by
+ *                        hand-made KATs of the remainder edge cases
+ *                        (negative coordinates in bin 0, m > size/64 where
+ *                        the if / else-if order picks the side) and by the
+ *                        input order inside every bin.  binner.cpp needs the
+ *                        CUDA runtime (cudaHostAlloc) and cannot be built
+ *                        here.
  *   orc_hogbom           textbook Hogbom CLEAN; the reference's deconvolve
  *                        (MS4/kernel/cpu/hogbom/hogbom.cpp:32-121) reads
  *                        out of bounds in findPeak and cannot serve as a
@@ -65,7 +78,10 @@
  *   7 w-plane from the scaled w
  *   8 normalisation by grid_size^2 instead of grid_size*grid_pitch
  *   9 phase rotation with the opposite sign
- *  10 Hogbom peak by signed value instead of magnitude */
+ *  10 Hogbom peak by signed value instead of magnitude
+ *  11 binner: right / top margin test not strict (rem >= size - margin)
+ *  12 binner: bins by floor division instead of C's truncating division
+ *  13 binner: first and last point swapped in the bin order */
 static int orc_mutation = 0;
 void orc_set_mutation(int m) { orc_mutation = m; }
 
@@ -466,4 +482,115 @@ unsigned orc_hogbom(double *mod, double *res, const double *psf, int siz,
     mod[(size_t)rx * pitch + ry] += a;
   }
   return it;
+}
+
+/* MS4/kernel/cpu/binner/binner.cpp:100-189, doit<OVER, DIVIDERS = 32,
+ * mirror> (bin / binm), with pregridPoint<8, mirror> compiled under __BINNER
+ * (CH:96-171: no translation inside the pregrid).  For every point in input
+ * order:
+ *   - p = the untranslated pregrid record; (qu, ru) = div(p.u, grid_size/32)
+ *     in C's truncating division, likewise v (binner.cpp:135-137);
+ *   - supp = gcfsupps[w_plane] (signed plane, centred table); the record is
+ *     translated by grid_size/2 - supp/2 and carries supp in place of the
+ *     plane (binner.cpp:143-151); the amplitude is rotated by
+ *     exp(2 pi i w scale) (binner.cpp:157-158, CH:90-94);
+ *   - it goes to bin (qu, qv), then, if margin = supp/2 > 0, to at most one
+ *     horizontal neighbour, one vertical neighbour and the diagonal between
+ *     them (binner.cpp:161-196; the if / else-if order decides the side).
+ * The reference appends to one stream per bin, so each bin holds its copies
+ * in input order (binner.cpp:87-90).
+ *
+ * Output: counts[32*32] (bin qu*32 + qv); if pre_out is not NULL, for each
+ * copy in bin order its record (pre_out, 4 int16: u, v, layer index,
+ * support), rotated amplitude (vis_out) and source point (src_out), up to
+ * cap copies.  Returns the number of copies, or -1 where the reference fails
+ * its assert (a home bin outside [0, 32)^2, binner.cpp:153-155), where a
+ * coordinate leaves int16 (undefined there) or grid_size < 32 (a division by
+ * zero there). */
+#define ORC_DIVS 32
+typedef struct { int bin[4]; short pre[4]; ocd vis; } orc_binpt;
+
+static int orc_bin_point(int mirror, double scale, double wstep, const ocd *amp,
+                         const od3 *uvw, const int *gcfsupps, int grid_size,
+                         orc_binpt *o)
+{
+  orc_pg p;
+  orc_pregrid(scale, wstep, uvw, 0, 0, mirror, &p);  /* __BINNER: untranslated */
+  if (!p.valid) return -1;
+  const int ds = grid_size / ORC_DIVS;
+  /* div(): quotient truncated toward zero, remainder of the dividend's sign */
+  const int qu = p.u / ds, ru = p.u % ds;
+  const int qv = p.v / ds, rv = p.v % ds;
+  if (orc_mutation == 12) {  /* planted: floor division */
+    const int fqu = (int)floor((double)p.u / ds), fqv = (int)floor((double)p.v / ds);
+    if (fqu != qu || fqv != qv) return -1;
+  }
+  if (qu < 0 || qv < 0 || qu >= ORC_DIVS || qv >= ORC_DIVS) return -1;
+  const int supp = gcfsupps[p.w_plane];
+  o->pre[0] = (short)(p.u + (short)(grid_size / 2 - supp / 2));
+  o->pre[1] = (short)(p.v + (short)(grid_size / 2 - supp / 2));
+  o->pre[2] = (short)p.layer_index;
+  o->pre[3] = (short)supp;
+  const double ph = 2.0 * M_PI * (uvw->w * scale);
+  const double s = sin(ph), c = cos(ph);
+  o->vis.re = amp->re * c - amp->im * s;
+  o->vis.im = amp->re * s + amp->im * c;
+  o->bin[0] = qu * ORC_DIVS + qv;
+  o->bin[1] = o->bin[2] = o->bin[3] = -1;
+  const int margin = supp / 2;
+  if (margin > 0) {
+    int du = 0, dv = 0;
+    const int right = orc_mutation == 11 ? ru >= ds - margin : ru > ds - margin;
+    if (ru < margin && qu >= 1) du = -1;
+    else if (right && qu < ORC_DIVS - 1) du = 1;
+    if (rv < margin && qv >= 1) dv = -1;
+    else if (rv > ds - margin && qv < ORC_DIVS - 1) dv = 1;
+    if (du) o->bin[1] = (qu + du) * ORC_DIVS + qv;
+    if (dv) o->bin[2] = qu * ORC_DIVS + qv + dv;
+    if (du && dv) o->bin[3] = (qu + du) * ORC_DIVS + qv + dv;
+  }
+  return 0;
+}
+
+long long orc_bin(int mirror, int n, double scale, double wstep, const ocd *amps,
+                  const od3 *uvws, const int *gcfsupps, int grid_size, int *counts,
+                  short *pre_out, ocd *vis_out, int *src_out, long long cap)
+{
+  const int nb = ORC_DIVS * ORC_DIVS;
+  for (int b = 0; b < nb; b++) counts[b] = 0;
+  if (grid_size < ORC_DIVS) return -1;
+  orc_binpt *pts = (orc_binpt *)malloc(sizeof(orc_binpt) * (size_t)(n > 0 ? n : 1));
+  if (!pts) return -1;
+  long long total = 0;
+  for (int i = 0; i < n; i++) {
+    if (orc_bin_point(mirror, scale, wstep, amps + i, uvws + i, gcfsupps, grid_size,
+                      pts + i) != 0) {
+      free(pts);
+      return -1;
+    }
+    for (int k = 0; k < 4; k++)
+      if (pts[i].bin[k] >= 0) { counts[pts[i].bin[k]]++; total++; }
+  }
+  if (pre_out) {
+    long long *cur = (long long *)malloc(sizeof(long long) * nb);
+    long long run = 0;
+    for (int b = 0; b < nb; b++) { cur[b] = run; run += counts[b]; }
+    for (int i = 0; i < n; i++) {
+      /* planted 13: the last point of each bin first */
+      const int ii = orc_mutation == 13 && i == 0 ? n - 1
+                     : orc_mutation == 13 && i == n - 1 ? 0 : i;
+      for (int k = 0; k < 4; k++) {
+        const int b = pts[ii].bin[k];
+        if (b < 0) continue;
+        const long long at = cur[b]++;
+        if (at >= cap) continue;
+        memcpy(pre_out + 4 * at, pts[ii].pre, sizeof(pts[ii].pre));
+        vis_out[at] = pts[ii].vis;
+        src_out[at] = ii;
+      }
+    }
+    free(cur);
+  }
+  free(pts);
+  return total;
 }

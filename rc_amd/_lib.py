@@ -117,6 +117,9 @@ SIGNATURES: dict[str, tuple] = {
     "mkGCFLayer": (vp, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
     "rcg_gcf_pad_quirk": (None, [C.c_int]),
     "calcAccums": (None, [vp, vp, vp, C.c_double, C.c_int, C.c_int]),
+    "bin": (vp, [C.c_int, C.c_double, C.c_double, vp, vp, vp, C.c_int]),
+    "binm": (vp, [C.c_int, C.c_double, C.c_double, vp, vp, vp, C.c_int]),
+    "rcg_bin_free": (None, [vp]),
     # MS6 Halide buffer_t entry points (buffer_t passed by pointer)
     "kern_scatter": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp]),
     "kern_degrid": (C.c_int, [C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]),

@@ -138,3 +138,66 @@ def deconvolve(res: np.ndarray, psf: np.ndarray, niters: int, gain: float,
                           niters, gain, threshold)
     _raise_on_error("deconvolve")
     return model
+
+
+class BinnedVis:
+    """The block bin / binm return, copied out of its pinned host memory.
+
+    counts[qu, qv] copies per bin; vis_off / pre_off[qu, qv] byte offsets
+    from the end of the table (the reference's binTable,
+    MS4/kernel/cpu/binner/binner.cpp:18-28); data_size the whole block's
+    bytes; vis (total,) complex128 and pre (total, 4) int16 records
+    (u, v, layer index, support) in bin order."""
+
+    def __init__(self, table: np.ndarray, data: bytes):
+        n = _BIN_DIVS * _BIN_DIVS
+        self.data_size = int(table[0])
+        self.data_off = int(table[1])
+        self.vis_off = table[2:2 + n].reshape(_BIN_DIVS, _BIN_DIVS).copy()
+        self.pre_off = table[2 + n:2 + 2 * n].reshape(_BIN_DIVS, _BIN_DIVS).copy()
+        self.counts = table[2 + 2 * n:2 + 3 * n].reshape(_BIN_DIVS, _BIN_DIVS).copy()
+        total = int(self.counts.sum())
+        self.raw = data
+        self.vis = np.frombuffer(data, np.complex128, count=total).copy()
+        self.pre = np.frombuffer(data, np.int16, count=4 * total,
+                                 offset=16 * total).reshape(total, 4).copy()
+
+    def bin(self, qu: int, qv: int) -> tuple[np.ndarray, np.ndarray]:
+        """(vis, pre) of one bin, read through the table's byte offsets."""
+        n = int(self.counts[qu, qv])
+        vo, po = int(self.vis_off[qu, qv]), int(self.pre_off[qu, qv])
+        vis = np.frombuffer(self.raw, np.complex128, count=n, offset=vo)
+        pre = np.frombuffer(self.raw, np.int16, count=4 * n, offset=po).reshape(n, 4)
+        return vis, pre
+
+
+_BIN_DIVS = 32
+BIN_TABLE_BYTES = 12320
+
+
+def bin_points(uvw: np.ndarray, vis: np.ndarray, scale: float, wstep: float,
+               gcf_supps: np.ndarray, max_w_plane: int, grid_size: int,
+               mirror: bool = False) -> BinnedVis:
+    """bin (mirror False) / binm (True) over host arrays: uvw (n, 3) float64,
+    vis (n,) complex128, gcf_supps the 2*max_w_plane + 1 supports of planes
+    -max_w_plane..max_w_plane (the callee indexes its centre)."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex128)
+    sup = np.ascontiguousarray(gcf_supps, np.int32)
+    if uvw.shape != (len(vis), 3) or len(sup) != 2 * max_w_plane + 1:
+        raise ValueError("uvw must be (n, 3), gcf_supps 2*max_w_plane + 1 long")
+    h = _lib.lib()
+    fn = h.binm if mirror else h.bin
+    centre = sup.ctypes.data + 4 * max_w_plane
+    p = fn(len(vis), scale, wstep, vis.ctypes.data, uvw.ctypes.data, centre, grid_size)
+    if not p:
+        code, msg = _lib.last_error()
+        raise _lib.RcgError(f"{'binm' if mirror else 'bin'}: {msg}")
+    try:
+        base = p - BIN_TABLE_BYTES
+        table = np.frombuffer(C.string_at(base, BIN_TABLE_BYTES), np.int32)
+        size = int(table[0])
+        data = C.string_at(p, size - BIN_TABLE_BYTES)
+    finally:
+        h.rcg_bin_free(p)
+    return BinnedVis(table, data)

@@ -2,9 +2,11 @@
 (oracle/rc_oracle.c, orc_set_mutation) must make at least one of the
 oracle's pinning checks fail: the golden vectors of the compiled reference,
 the pregrid probe, the small-kernel vectors, the strict in-bound KAT, the
-transpose identity and the Hogbom brute-force checks."""
+transpose identity, the Hogbom brute-force checks and the binner's
+geometric, KAT and order checks."""
 import pytest
 
+import test_oracle_binner as TB
 import test_oracle_golden as TG
 import test_oracle_math as TM
 from _fixtures import RANDOM_FIXTURES
@@ -17,7 +19,9 @@ CHECKS = (
        ("strict in-bound KAT", TM.test_out_of_bound_strict_upper_limit),
        ("transpose identity", lambda: TM.test_degrid_is_the_transpose(True)),
        ("hogbom point source", TM.test_hogbom_recovers_point_source_exactly),
-       ("hogbom negative peaks", TM.test_hogbom_residual_identity_and_negative_peaks)]
+       ("hogbom negative peaks", TM.test_hogbom_residual_identity_and_negative_peaks),
+       ("binner copies and order", TB.test_records_copies_and_order),
+       ("binner remainder KATs", TB.test_remainder_edge_kats)]
 )
 
 MUTATIONS = {
@@ -31,6 +35,9 @@ MUTATIONS = {
     8: "normalisation by size^2, not size*pitch",
     9: "phase rotation sign flipped",
     10: "Hogbom peak by signed value",
+    11: "binner right margin not strict",
+    12: "binner floor division",
+    13: "binner bin order broken",
 }
 
 
