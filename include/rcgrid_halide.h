@@ -138,6 +138,39 @@ void gridGPUs(double scale, double *out, double *in, double *in_vals, size_t npt
  *                    image(x, y);
  * where clamp() limits to the output's dimension-0 range [left, right], for
  * y as well (the reference clamps y with the x bounds). */
+/* The MS6 visibility binning kernels (MS6/programs/Kernel/Binning.hs), which
+ * are Haskell loops over host vectors in the reference (no C counterpart);
+ * these entry points run the per-point work on the GPU for a Haskell kernel
+ * to call.  Regions, boxes and bins are in uv units (gridXY2UV applied,
+ * MS6/programs/Kernel/Config.hs:211-212); vis holds n rows of `width`
+ * doubles (u, v, w, re, im, ...).
+ *   rcg_flow_bin_sizes  binSizer (:34-97): low, high = min and max of 0 and
+ *     the w of the points inside the regions' bounding box, high +=
+ *     (high - low) 1e-4; for region r (4 doubles: u_low, u_high, v_low,
+ *     v_high) and bin b, out[(r bins + b) 3 + {0, 1, 2}] = start, end (as
+ *     doubles) and count (as int64) with start = low + b (high - low) / bins;
+ *     a point is counted in the region with the greatest u_low <= u, then
+ *     the greatest v_low <= v (no upper test, as Data.Map.lookupLE), bin
+ *     floor((w - low) / (high - low) bins) clamped to [0, bins); low_high
+ *     (optional) receives low and high.  A NaN w inside the box is an error
+ *     (the reference's fold result would depend on its position).
+ *   rcg_flow_binner     binner (:99-181), width 5: box b (4 doubles) owns the
+ *     bins [bin_off[b], bin_off[b + 1]) with ranges bin_range[2 k], [2 k + 1]
+ *     and sizes bin_size[k]; out[b] holds their summed sizes in rows of 5
+ *     doubles.  Every point goes, in input order, to the box found by the
+ *     greatest w_low <= w, then v_low <= v, then u_low <= u (the first box
+ *     and bin with a given triple of lows wins) if it lies in that box's
+ *     half-open ranges; then rows [filled, size) are zeroed for every bin of
+ *     the box.  filled (optional) receives the rows written per box.  A box
+ *     that would receive more rows than it holds is an error (the reference
+ *     writes past its vector).
+ * Return 0, or -1 with rcg_last_error() set. */
+int rcg_flow_bin_sizes(const double *vis, int n, int width, const double *regions,
+                       int n_regions, int bins, void *out, double *low_high);
+int rcg_flow_binner(const double *vis, int n, int width, const double *boxes,
+                    int n_boxes, const int *bin_off, const double *bin_range,
+                    const long long *bin_size, double *const *out, long long *filled);
+
 int kern_init(buffer_t *grid);
 int kern_image_init(buffer_t *image);
 int kern_psf_vis(buffer_t *vis, buffer_t *out);

@@ -138,6 +138,8 @@ SIGNATURES: dict[str, tuple] = {
     "kern_detile": (C.c_int, [vp, vp]),
     "kern_defacet": (C.c_int, [vp, vp]),
     "kern_image_sum": (C.c_int, [vp, vp]),
+    "rcg_flow_bin_sizes": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp, vp]),
+    "rcg_flow_binner": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]),
     # GCF generation (device)
     "rcg_gcf_gen_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_gcf_gen_layer": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
