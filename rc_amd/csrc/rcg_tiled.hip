@@ -702,19 +702,19 @@ __global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restri
 // wavefront four pairs at a time.
 
 // Sums of 4 pairs at once.  Pair k of a wavefront (in the order it walks
-// them) has its output address written to meta[k % 8] by lane 0 when the
-// pair is decoded (two halves of 4, so the next pair's address never
-// overwrites one still to be used), and when it ends every lane leaves its
-// part of the pair's sum in row k % 4 of the wavefront's LDS buffer.  After
+// them) has its output code in codes[k % 128], written by lane k % 64 when
+// its batch of 64 records becomes current, and when it ends every lane
+// leaves its part of the pair's sum in row k % 4 of the wavefront's LDS
+// buffer.  After
 // pairs 4a..4a+3 (or at the end of an item) lane l sums entries c, c + 16,
 // c + 32, c + 48 of row l / 16 (c = l % 16), then a DPP prefix sum over its
 // row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill) leaves the pair's
 // total in the row's last lane, which stores it.  The order of the
 // additions is fixed, so the result is reproducible.
 constexpr int SUMS = 4;
-// Output addresses kept in LDS are typed global, so the final store is a
-// global (not flat) store and does not hold up the wavefront's LDS waits;
-// the (re, im) pair is stored as one 64-bit word, re in the low half.
+// Output addresses are typed global, so the final store is a global (not
+// flat) store and does not hold up the wavefront's LDS waits; the (re, im)
+// pair is stored as one 64-bit word, re in the low half.
 typedef __attribute__((address_space(1))) unsigned long long gu64;
 
 __device__ __forceinline__ float row_sum(float x)
@@ -730,10 +730,20 @@ __device__ __forceinline__ float row_sum(float x)
   return x;
 }
 
-// n: pairs in the buffer (1..SUMS); entries of the other rows are stale
-// and their sums are not stored
-__device__ __forceinline__ void flush_sums(const float2 *buf, gu64 *const *meta, int n,
-                                           int lane)
+// Output of a pair, from its record: ~point for a point whose window lies
+// in one tile (its visibility), slot * npts + point for a split point (its
+// partial); below 2^31 because plans hold at most 2^31 - 1 pairs.
+__device__ __forceinline__ int out_code(const int4 &rec, int npts)
+{
+  const int whole = rec.x >> 31;  // all ones for a whole pair
+  const int split = ((rec.x >> 24) & 63) * npts + rec.w;
+  return (~rec.w & whole) | (split & ~whole);
+}
+
+// n: pairs in the buffer (1..SUMS), whose output codes are code[0..n);
+// entries of the other rows are stale and their sums are not stored
+__device__ __forceinline__ void flush_sums(const float2 *buf, const int *code, float2 *vis,
+                                           float2 *partial, int n, int lane)
 {
   // the entries come from other lanes of the wavefront: keep the compiler
   // from moving LDS accesses across this point (the hardware runs a
@@ -744,8 +754,11 @@ __device__ __forceinline__ void flush_sums(const float2 *buf, gu64 *const *meta,
   const float2 a0 = b[0], a1 = b[16], a2 = b[32], a3 = b[48];
   const float re = row_sum((a0.x + a1.x) + (a2.x + a3.x));
   const float im = row_sum((a0.y + a1.y) + (a2.y + a3.y));
-  if (c == 15 && q < n)
-    *meta[q] = (unsigned long long)__float_as_uint(im) << 32 | __float_as_uint(re);
+  if (c == 15 && q < n) {
+    const int o = code[q];
+    gu64 *dst = (gu64 *)(o < 0 ? vis + ~o : partial + o);
+    *dst = (unsigned long long)__float_as_uint(im) << 32 | __float_as_uint(re);
+  }
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -759,12 +772,15 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
   constexpr int DUMMY = T * LS;
   __shared__ float2 tile[T * LS + 2];
   __shared__ float2 sums[4][SUMS * 64];
-  __shared__ gu64 *meta[4][2 * SUMS];
+  // output codes of the wavefront's pairs k mod 128: two batches of 64, the
+  // current one and the one whose last sums may still be pending
+  __shared__ int codes[4][128];
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int npts = (int)d.npts;
   float2 *my_sums = sums[wave];
-  gu64 **my_meta = meta[wave];
+  int *my_codes = codes[wave];
   for_each_item(d, d.queue + 8, [&](const ItemRange &ir) {
     const int gsz = d.grid_size;
     const float2 *gi = grid + ir.goff;
@@ -783,11 +799,8 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm, 4);
       stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n, 4);
       int k = 0, s0 = 0;
+      my_codes[lane] = out_code(rec, npts);
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
-      auto out_addr = [&](const Pair &q) {
-        return (gu64 *)(q.whole ? vis + q.point : d.partial + (long long)q.slot * d.npts + q.point);
-      };
-      if (lane == 0) my_meta[0] = out_addr(p);
       float sg = p.conj ? -1.f : 1.f;
       float2 ta[GRP], tb[GRP];
       int la[GRP], lb[GRP];
@@ -812,12 +825,14 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
           if (kn < npw) {
             const int j = kn & 63;
             if (j == 0) {
+              // the slots of this batch last held the batch before the
+              // previous one, whose sums are all stored
               rec = rec_n;
+              my_codes[(kn & 64) + lane] = out_code(rec, npts);
               stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n, 4);
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  0);
-            if (lane == 0) my_meta[kn & (2 * SUMS - 1)] = out_addr(pn);
             sgn = pn.conj ? -1.f : 1.f;
           } else {
             // past the end: the empty pair, no lane active; it never ends,
@@ -843,7 +858,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
           // k_degrid_reduce
           my_sums[(k & (SUMS - 1)) * 64 + lane] = make_float2(fmaf(-sg, Q, P), fmaf(sg, R, U));
           if ((k & (SUMS - 1)) == SUMS - 1)
-            flush_sums(my_sums, my_meta + (k & SUMS), SUMS, lane);
+            flush_sums(my_sums, my_codes + (k & 124), vis, d.partial, SUMS, lane);
           P = Q = R = U = 0.f;
         }
         p = pn;
@@ -860,7 +875,8 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
         body(tb, lb, ta, la);
       } while (more);
       if (npw & (SUMS - 1))
-        flush_sums(my_sums, my_meta + ((npw - 1) & SUMS), npw & (SUMS - 1), lane);
+        flush_sums(my_sums, my_codes + ((npw - 1) & 124), vis, d.partial, npw & (SUMS - 1),
+                   lane);
       // never true (chunk >= 1): keeps the prefetch loads from sinking
       if (d.chunk < 0) {
         float2 z = make_float2(0.f, 0.f);
