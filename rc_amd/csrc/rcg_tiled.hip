@@ -46,10 +46,10 @@ namespace rcg {
 //      slot:   index of this tile among the point's tiles (row-major)
 //      whole:  the window lies entirely in this tile
 //   y: tap offset (elements) of the intersection's first cell
-//   z: stride | nfull << 16 | hrem << 23
+//   z: stride | h << 16 | nslabs << 23
 //      stride: layer row stride (elements, < 65536)
-//      nfull:  slabs whose rows are all inside (h / rows)
-//      hrem:   rows of the last, partial slab (h % rows)
+//      h:      intersection height (1..64)
+//      nslabs: 64-lane slabs of `rows` rows that cover it, ceil(h / rows)
 //   w: point index
 __device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int point,
                                           int tile_r, int tile_c, int T, int slot)
@@ -60,14 +60,14 @@ __device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int p
   const int ca = max(v0, tv0), cb = min(v0 + S, tv0 + T);
   const int h = rb - ra, w = cb - ca;
   const int rows = 64 / w;
-  const int nfull = h / rows, hrem = h - nfull * rows;
+  const int nslabs = (h + rows - 1) / rows;
   const int whole = (h == S) && (w == S);
   int4 p;
   p.x = (ra - tu0) | ((ca - tv0) << 6) | ((w - 1) << 12) | ((rows - 1) << 18) |
         (slot << 24) | ((r.flags & PG_CONJ) ? (1 << 30) : 0) |
         (whole ? (int)0x80000000 : 0);
   p.y = r.tap0 + (ra - u0) * stride + (ca - v0);
-  p.z = stride | (nfull << 16) | (hrem << 23);
+  p.z = stride | (h << 16) | (nslabs << 23);
   p.w = point;
   return p;
 }
@@ -451,14 +451,12 @@ __device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int 
 {
   Pair p;
   const int stride = pz & 0xffff;
-  const int nfull = (pz >> 16) & 127;
-  const int hrem = (pz >> 23) & 63;
   p.point = pw;
   const int r0 = px & 63, c0 = (px >> 6) & 63;
   const int w = ((px >> 12) & 63) + 1;
   p.rows = ((px >> 18) & 63) + 1;
-  p.h = nfull * p.rows + hrem;
-  p.nslabs = nfull + (hrem ? 1 : 0);
+  p.h = (pz >> 16) & 127;
+  p.nslabs = (pz >> 23) & 127;
   p.conj = (px >> 30) & 1;
   p.whole = px < 0;
   p.slot = (px >> 24) & 63;
