@@ -468,7 +468,7 @@ __device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int 
   const bool idle = lr >= p.rows;
   p.lr = idle ? 64 : lr;
   p.tb0 = idle ? TAP_OOB : (unsigned)(py + __mul24(lr, stride) + lc) * 8u;
-  p.lb0 = base + ((r0 + lr) * LS + c0 + lc) * 8;
+  p.lb0 = base + (__mul24(r0 + lr, LS) + c0 + lc) * 8;
   p.tstep = p.rows * stride * 8;
   p.lstep = p.rows * LS * 8;
   return p;
