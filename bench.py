@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision", "binner",
-                                           "flow"],
+                                           "flow", "small"],
                     default="wproj",
                     help="wproj: the 4096^2 w-projection workload (default); batched: 64 "
                          "uneven 2048^2 snapshot instances, one graph launch vs one launch "
@@ -53,7 +53,8 @@ def parse():
                          "precision: 50M points of the w-projection layout, FP32 with and "
                          "without compensated flushes against the FP64 path; binner: the "
                          "MS4 uv binner (binm) through its host-memory C ABI; flow: the "
-                         "MS6 binSizer + binner kernels")
+                         "MS6 binSizer + binner kernels; small: normalize, rotate, grid0, "
+                         "reweight, hermitian fold and FFT against their rooflines")
     ap.add_argument("--instances", type=int, default=64)
     return ap.parse_args()
 
@@ -533,6 +534,114 @@ def bench_flow(args) -> int:
     return 0
 
 
+def bench_small(args) -> int:
+    """The small kernels next to the gridder on the headline sizes (4096^2
+    complex128 grid, the 9,987,840 points of the w-projection workload):
+    normalize, rotate, grid0, reweight, the hermitian fold and the
+    fft_inplace_even transform, each timed with HIP events on the stream it
+    runs on, against its algorithmic bytes (HBM roofline, 8 TB/s).  The CPU
+    baselines are the reference's own functions compiled with its release
+    flags (normalizeCPU with OpenMP, the others serial as in the reference),
+    the oracle's hermitian fold and numpy's FFT."""
+    import ctypes as C
+    import torch
+    from rc_amd import _lib, synth
+    from rc_amd import device as D
+
+    vc, _, N = synth.wproj(n_times=args.n_times)
+    n, P = vc.n, N
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    uvw_h = np.ascontiguousarray(vc.uvw.reshape(-1, 3), np.float64)
+    vis_h = vc.vis.reshape(-1).astype(np.complex128)
+    grid_h = (rng.standard_normal((N, P)) + 1j * rng.standard_normal((N, P))).astype(np.complex128)
+    uvw = torch.from_numpy(uvw_h).to(dev)
+    vis = torch.from_numpy(vis_h).to(dev)
+    grid = torch.from_numpy(grid_h).to(dev)
+    counts = torch.empty(N * N, dtype=torch.int32, device=dev)
+    h = _lib.lib()
+    st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+    fft = D.FFTPlan(-1, N, P, f64=True)
+    scale = vc.scale
+
+    def run(name, fn):
+        _lib.check(fn(), name)
+
+    kernels = {
+        # name: (call, algorithmic bytes per launch)
+        "normalize": (lambda: h.rcg_normalize(grid.data_ptr(), 1, P, N, sp), 32 * N * P),
+        "rotate": (lambda: h.rcg_rotate(uvw.data_ptr(), vis.data_ptr(), 1, n, scale, sp),
+                   (24 + 32) * n),
+        "grid0": (lambda: h.rcg_grid0(uvw.data_ptr(), vis.data_ptr(),
+                                      grid.data_ptr() + (N // 2 * (P + 1)) * 16, scale, n, P, N,
+                                      sp), (24 + 16 + 32) * n),
+        "reweight": (lambda: h.rcg_reweight(uvw.data_ptr(), vis.data_ptr(), scale, n, N,
+                                            counts.data_ptr(), sp),
+                     (24 + 8) * n + (24 + 4 + 32) * n + 4 * N * N),
+        "herm_padded": (lambda: h.rcg_herm_padded(grid.data_ptr(), N, P, sp), 24 * N * P),
+        "fft_c2c": (lambda: h.rcg_fft_execute(fft._h, grid.data_ptr(), sp), 3 * 32 * N * P),
+    }
+    res = {}
+    for name, (fn, nbytes) in kernels.items():
+        for _ in range(args.warmup):
+            run(name, fn)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(args.steps):
+            run(name, fn)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        gbs = nbytes / (ms / 1e3) / 1e9
+        res[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 3)}
+    fft.close()
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(16, len(os.sched_getaffinity(0)))
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        ref = O.ref_lib(fast=True) if O.ref_available(fast=True) else None
+        g = grid_h.copy()
+        v = vis_h.copy()
+
+        def cpu(name, fn, kind, cores):
+            t = time.perf_counter()
+            fn()
+            res[name]["cpu_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+            res[name]["cpu_kind"] = kind
+            res[name]["cpu_cores"] = cores
+            res[name]["speedup"] = round(res[name]["cpu_ms"] / res[name]["ms"], 1)
+
+        if ref is not None:
+            cpu("normalize", lambda: ref.normalizeCPU(g.ctypes.data, P, N), "reference", threads)
+            cpu("rotate", lambda: ref.rotateCPU(uvw_h.ctypes.data, v.ctypes.data, n, scale),
+                "reference", 1)
+            cpu("grid0", lambda: ref.grid0(uvw_h.ctypes.data, v.ctypes.data,
+                                           g.ctypes.data + (N // 2 * (P + 1)) * 16, scale, n, P,
+                                           N), "reference", 1)
+            cpu("reweight", lambda: ref.reweight(uvw_h.ctypes.data, v.ctypes.data, scale, n, N),
+                "reference", 1)
+        cpu("herm_padded", lambda: O.oracle_lib().orc_herm_padded(g.ctypes.data, N, P), "port",
+            1)
+        cpu("fft_c2c", lambda: np.fft.fft2(g), "port (numpy)", 1)
+    fr = [r["frac"] for r in res.values()]
+    line = {
+        "metric": "small kernels on the headline sizes: geometric-mean fraction of the HBM "
+                  "roofline", "value": round(float(np.exp(np.mean(np.log(fr)))), 3),
+        "unit": "fraction of 8 TB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sum(r["ms"] for r in res.values()), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "4096x4096 complex128 grid, 9,987,840 w-projection points",
+                   "points": n, "grid": N},
+        "kernels": res,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -557,6 +666,8 @@ def main() -> int:
         return bench_binner(args)
     if args.workload == "flow":
         return bench_flow(args)
+    if args.workload == "small":
+        return bench_small(args)
     dev = torch.device("cuda:0")
     t0 = time.time()
     vc, gcf, N = synth.wproj(n_times=args.n_times)
