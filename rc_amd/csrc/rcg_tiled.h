@@ -30,6 +30,7 @@ struct TiledDev {
   long long gstride; // grid elements per instance (grid_size * pitch)
   long long pair_cap;
   long long npts;    // points of the plan (all instances)
+  double inv_pitch;  // 1 / pitch
   int *tile_count;   // pairs per tile
   int *tile_start;   // first pair of each tile
   int *tile_cursor;  // fill cursors

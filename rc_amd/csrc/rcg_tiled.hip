@@ -59,8 +59,11 @@ __device__ __forceinline__ int4 make_pair(int u0, int v0, const rcg_pg &r, int p
   const int ra = max(u0, tu0), rb = min(u0 + S, tu0 + T);
   const int ca = max(v0, tv0), cb = min(v0 + S, tv0 + T);
   const int h = rb - ra, w = cb - ca;
-  const int rows = 64 / w;
-  const int nslabs = (h + rows - 1) / rows;
+  // 64 / w and ceil(h / rows) through the approximate reciprocal: (x + 0.5)
+  // / y lies at least 0.5 / y >= 1/128 from an integer for x < 128, y <= 64,
+  // far above the reciprocal's error
+  const int rows = (int)(64.5f * __builtin_amdgcn_rcpf((float)w));
+  const int nslabs = (int)(((float)(h + rows - 1) + 0.5f) * __builtin_amdgcn_rcpf((float)rows));
   const int whole = (h == S) && (w == S);
   int4 p;
   p.x = (ra - tu0) | ((ca - tv0) << 6) | ((w - 1) << 12) | ((rows - 1) << 18) |
@@ -102,11 +105,14 @@ __global__ void k_fill_f32(float *__restrict__ p, long long n)
 
 constexpr int BIN_THREADS = 1024;
 
-__device__ __forceinline__ void point_tiles(const rcg_pg &r, int pitch, int lt,
+// u0 = cell / pitch without an integer division: (cell + 0.5) / pitch lies
+// at least 0.5 / pitch (>= 2^-16) from an integer, and the FP64 product
+// (cell < 2^31) is within 2^-21 of it.
+__device__ __forceinline__ void point_tiles(const rcg_pg &r, int pitch, double inv_pitch, int lt,
                                             int *u0, int *v0, int *tu0,
                                             int *tu1, int *tv0, int *tv1)
 {
-  *u0 = r.cell / pitch;
+  *u0 = (int)(((double)r.cell + 0.5) * inv_pitch);
   *v0 = r.cell - *u0 * pitch;
   const int S = r.supp;
   *tu0 = *u0 >> lt;
@@ -150,7 +156,7 @@ k_bin_count(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
     const rcg_pg r = pg[i];
     if (r.cell < 0 || r.supp <= 0) continue;
     int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
     if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) {
       atomicOr(err, ERR_CAPACITY);  // support above the plan's max_support
       continue;
@@ -265,7 +271,7 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
     const rcg_pg r = pg[i];
     if (r.cell < 0 || r.supp <= 0) continue;
     int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
     if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) continue;
     int j = 0;
     for (int a = tu0; a <= tu1; a++)
@@ -286,7 +292,7 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
     const rcg_pg r = pg[i];
     if (r.cell < 0 || r.supp <= 0) continue;
     int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
     if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) continue;
     int j = 0;
     for (int a = tu0; a <= tu1; a++)
@@ -901,7 +907,7 @@ k_degrid_reduce(TiledDev d, const rcg_pg *__restrict__ pg, long long n,
   const rcg_pg r = pg[i];
   if (r.cell < 0 || r.supp <= 0) return;
   int u0, v0, tu0, tu1, tv0, tv1;
-  point_tiles(r, d.pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+  point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
   const int np = (tu1 - tu0 + 1) * (tv1 - tv0 + 1);
   if (np <= 1 || np > d.mpp) return;  // whole (written by the tile kernel)
   // partials are stored slot-major, so consecutive points read consecutive
@@ -1083,6 +1089,7 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
   d.gstride = (long long)size * pitch;
   d.pair_cap = n * max_pairs_pt;
   d.npts = n;
+  d.inv_pitch = 1.0 / pitch;
   const int n_seg = ceil_div(d.n_tiles, SCAN_SEG);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bin_fill),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
