@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision", "binner",
-                                           "flow", "small"],
+                                           "flow", "small", "next"],
                     default="wproj",
                     help="wproj: the 4096^2 w-projection workload (default); batched: 64 "
                          "uneven 2048^2 snapshot instances, one graph launch vs one launch "
@@ -54,7 +54,8 @@ def parse():
                          "without compensated flushes against the FP64 path; binner: the "
                          "MS4 uv binner (binm) through its host-memory C ABI; flow: the "
                          "MS6 binSizer + binner kernels; small: normalize, rotate, grid0, "
-                         "reweight, hermitian fold and FFT against their rooflines")
+                         "reweight, hermitian fold and FFT against their rooflines; next: "
+                         "GCF generation, MS6 scatter/degrid and nvGridder")
     ap.add_argument("--instances", type=int, default=64)
     return ap.parse_args()
 
@@ -642,6 +643,123 @@ def bench_small(args) -> int:
     return 0
 
 
+def bench_next(args) -> int:
+    """The components beside the path, on the headline points:
+    - GCF generation (the reference's GCF kernel: per-plane mean w, then one
+      mkGCFLayer per plane; planes -31..31, supports min(64, 8 + 2|p|))
+      on the device, against the oracle's numpy restatement (the reference's
+      GCF.cpp needs FFTW, which is absent);
+    - the MS6 buffer_t scatter and degrid (FP64, S = 16, the full 4096^2
+      grid) and nvGridder (2048^2 grid, 16x16 GCF, with its in-place sort)
+      through their host-memory ABIs, against the oracle's per-point
+      restatements on a sample (Halide and CUDA are absent)."""
+    import ctypes as C
+    import torch
+    from oracle import gcf_oracle as GO
+    from oracle import halide_oracle as HO
+    from rc_amd import gcf as G
+    from rc_amd import halide as H
+    from rc_amd import synth
+
+    vc, _, N = synth.wproj(n_times=args.n_times)
+    n = vc.n
+    uvw_h = np.ascontiguousarray(vc.uvw.reshape(-1, 3), np.float64)
+    res = {}
+
+    def timed(fn, steps, warmup, sync=True):
+        for _ in range(warmup):
+            fn()
+        if sync:
+            torch.cuda.synchronize()
+        ts = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            fn()
+            if sync:
+                torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)) * 1e3
+
+    # GCF generation
+    uvw_d = torch.from_numpy(uvw_h).cuda()
+    M, theta = 31, 0.1
+    ms = timed(lambda: G.make_gcf(uvw_d, vc.wstep, theta, 8, 2, 64, max_w_plane=M),
+               max(3, args.steps // 2), 1)
+    r = {"ms": round(ms, 3), "planes": 2 * M + 1}
+    if not args.no_cpu:
+        t = time.perf_counter()
+        GO.gcf_kernel(uvw_h, vc.wstep, theta, 8, 2, 64, M)
+        r["cpu_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        r["cpu_kind"] = "port (oracle, numpy FFT)"
+        r["speedup"] = round(r["cpu_ms"] / ms, 1)
+    res["gcf_generation"] = r
+
+    # MS6 scatter / degrid, buffer_t ABI, FP64
+    rng = np.random.default_rng(7)
+    S = 16
+    vis5 = np.zeros((n, 5))
+    vis5[:, :3] = uvw_h
+    vis5[:, 3] = vc.vis.reshape(-1).real
+    vis5[:, 4] = vc.vis.reshape(-1).imag
+    gcf = np.ascontiguousarray(rng.standard_normal((64, S, S, 2)))
+    uvg = np.zeros((N, N, 2))
+    vb, gb, ub = H.vis_buffer(vis5), H.gcf_buffer(gcf), H.uvg_buffer(uvg)
+    out5 = np.zeros_like(vis5)
+    ob = H.vis_buffer(out5)
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} returned {rc}")
+
+    ms_s = timed(lambda: ok(H.scatter(vc.scale, N, 0, vb, gb, ub), "kern_scatter"), 3, 1,
+                 sync=False)
+    ms_d = timed(lambda: ok(H.degrid(vc.scale, N, 0, gb, ub, vb, ob), "kern_degrid"), 3, 1,
+                 sync=False)
+    r = {"points": n, "scatter_ms": round(ms_s, 1), "degrid_ms": round(ms_d, 1),
+         "scatter_mpts_s": round(n / ms_s / 1e3, 1), "degrid_mpts_s": round(n / ms_d / 1e3, 1)}
+    if not args.no_cpu:
+        m = 20_000
+        g2 = np.zeros((N, N, 2))
+        t = time.perf_counter()
+        HO.scatter(vis5[:m], gcf, g2, vc.scale, N, 0)
+        r["cpu_scatter_mpts_s"] = round(m / (time.perf_counter() - t) / 1e6, 4)
+        r["cpu_kind"] = f"port (oracle, Python loop over the first {m} points)"
+    res["ms6_scatter_degrid"] = r
+    del uvg
+
+    # nvGridder, 2048^2, 16x16 GCF
+    NG = 2048
+    m_nv = min(n, 2_000_000)
+    nv = np.zeros((m_nv, 5))
+    nv[:, 0] = uvw_h[:m_nv, 0] * vc.scale * (NG - 48) / N + NG / 2
+    nv[:, 1] = uvw_h[:m_nv, 1] * vc.scale * (NG - 48) / N + NG / 2
+    nv[:, 3:] = vis5[:m_nv, 3:]
+    gnv = np.ascontiguousarray(rng.standard_normal((64, 16, 16, 2)))
+    out_nv = np.zeros((NG, NG, 2))
+    nvb, ngb, nob = H.vis_buffer(nv), H.nv_gcf_buffer(gnv), H.uvg_buffer(out_nv)
+    ms_nv = timed(lambda: ok(H.nv_gridder(1.0, NG, nvb, ngb, nob), "nvGridder"), 3, 1,
+                  sync=False)
+    r = {"points": m_nv, "ms": round(ms_nv, 1), "mpts_s": round(m_nv / ms_nv / 1e3, 1)}
+    if not args.no_cpu:
+        m = 1000
+        t = time.perf_counter()
+        HO.nv_grid(nv[:m], gnv, 1.0, NG)
+        r["cpu_mpts_s"] = round(m / (time.perf_counter() - t) / 1e6, 5)
+        r["cpu_kind"] = f"port (oracle, Python loop over {m} points)"
+    res["nv_gridder"] = r
+    line = {
+        "metric": "next-row components on the headline points: GCF generation, MS6 buffer_t "
+                  "scatter/degrid, nvGridder (host-memory ABIs, PCIe included)",
+        "value": res["ms6_scatter_degrid"]["scatter_mpts_s"], "unit": "Mpoints/s (MS6 scatter)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "w-projection points (256 antennas x 306 times)", "points": n},
+        "components": res,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -668,6 +786,8 @@ def main() -> int:
         return bench_flow(args)
     if args.workload == "small":
         return bench_small(args)
+    if args.workload == "next":
+        return bench_next(args)
     dev = torch.device("cuda:0")
     t0 = time.time()
     vc, gcf, N = synth.wproj(n_times=args.n_times)
