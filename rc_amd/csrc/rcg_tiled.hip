@@ -561,78 +561,97 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       float2 mv, mv_n;
       stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv, GW);
       stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n, GW);
-      int k = 0, s0 = 0;
-      Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane,
-                               base);
-      float vx = rlf(mv.x, 0), vy = rlf(mv.y, 0);
-      float2 ta[GRP], tb[GRP];
-      int la[GRP], lb[GRP];
-      load_group(ta, la, p, 0, dummy, rsrc);
+      // The group after (q, sq): the same pair's next slabs, or the next
+      // pair's first; past the end, the empty pair (no lane active).
+      auto advance = [&](Pair &q, int &sq, int &kq, float &qx, float &qy) {
+        sq += GRP;
+        if (sq >= q.nslabs) {
+          kq++;
+          sq = 0;
+          if (kq < npw) {
+            const int j = kq & 63;
+            if (j == 0) {
+              rec = rec_n;
+              mv = mv_n;
+              stream_batch<true>(d, vis, ir, wave, lane, npw, (kq >> 6) + 1, rec_n, mv_n, GW);
+            }
+            q = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
+                                base);
+            qx = rlf(mv.x, j);
+            qy = rlf(mv.y, j);
+          } else {
+            q.h = 0;
+          }
+        }
+      };
+      // Taps are issued two groups ahead: the current group's were issued two
+      // groups ago, the next group's one group ago.  cur: the current
+      // group's visibility, conjugation and pair index; (pn, sn, kn): the
+      // next group, whose successor this group issues.
+      Pair pn = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane,
+                                base);
+      int sn = 0, kn = 0;
+      float vxn = rlf(mv.x, 0), vyn = rlf(mv.y, 0);
+      float2 ta[GRP], tb[GRP], tc3[GRP];
+      int la[GRP], lb[GRP], lc3[GRP];
+      load_group(ta, la, pn, sn, dummy, rsrc);
+      float vx = vxn, vy = vyn;
+      int cj = pn.conj, kc = 0;
+      advance(pn, sn, kn, vxn, vyn);
+      load_group(tb, lb, pn, sn, dummy, rsrc);
       bool more = true;
-      // One group: choose the next group (the same pair's next slabs, or the
-      // next pair's first; past the end, the empty pair), issue its taps into
-      // tn, then update the current group's cells from tc while they fly.  Called alternately with the two buffers, so no
-      // register copy forces a wait on the loads in flight.
-      auto body = [&](float2 (&tc)[GRP], int (&lc)[GRP], float2 (&tn)[GRP], int (&ln)[GRP]) {
+      // One group: issue the taps of the group after the next one into t2,
+      // then update the current group's cells from t0 while they fly.
+      // Called with the three buffers in rotation, so no register copy
+      // forces a wait on the loads in flight.
+      auto body = [&](float2 (&t0)[GRP], int (&l0)[GRP], float2 (&t2)[GRP], int (&l2)[GRP]) {
         // The current group's cells, read first so that the LDS latency
-        // overlaps the choice of the next group.  A wavefront's LDS
+        // overlaps the choice of the group after next.  A wavefront's LDS
         // operations complete in order, so the reads see the previous
         // group's writes; the cells of one group are distinct, so all its
         // reads may precede its writes.
         float2 x[GRP];
 #pragma unroll
-        for (int u = 0; u < GRP; u++) x[u] = lds_at(acc, lc[u]);
-        Pair pn = p;
-        float vxn = vx, vyn = vy;
-        int kn = k, sn = s0 + GRP;
-        if (sn >= p.nslabs) {
-          kn = k + 1;
-          sn = 0;
-          if (kn < npw) {
-            const int j = kn & 63;
-            if (j == 0) {
-              rec = rec_n;
-              mv = mv_n;
-              stream_batch<true>(d, vis, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, mv_n, GW);
-            }
-            pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
-                                 base);
-            vxn = rlf(mv.x, j);
-            vyn = rlf(mv.y, j);
-          } else {
-            pn.h = 0;  // past the end: the empty pair, no lane active
-          }
-        }
-        more = kn < npw;
-        load_group(tn, ln, pn, sn, dummy, rsrc);
+        for (int u = 0; u < GRP; u++) x[u] = lds_at(acc, l0[u]);
+        Pair pnn = pn;
+        int snn = sn, knn = kn;
+        float vxnn = vxn, vynn = vyn;
+        advance(pnn, snn, knn, vxnn, vynn);
+        load_group(t2, l2, pnn, snn, dummy, rsrc);
         __builtin_amdgcn_sched_barrier(0);
         // taps are conjugated for mirrored (w < 0) layers and, in the
         // adjoint, once more for all of them: b = cj ? vy : -vy and
         // c = cj ? -vx : vx, as sign-bit flips (scalar integer ops)
-        const unsigned flip = p.conj != d.adj ? 0u : 0x80000000u;
+        const unsigned flip = cj != d.adj ? 0u : 0x80000000u;
         const float a = vx, e = vy;
         const float b = __int_as_float(__float_as_int(vy) ^ flip);
         const float c = __int_as_float(__float_as_int(vx) ^ flip ^ 0x80000000u);
 #pragma unroll
         for (int u = 0; u < GRP; u++) {
-          x[u].x = fmaf(a, tc[u].x, fmaf(b, tc[u].y, x[u].x));
-          x[u].y = fmaf(c, tc[u].y, fmaf(e, tc[u].x, x[u].y));
+          x[u].x = fmaf(a, t0[u].x, fmaf(b, t0[u].y, x[u].x));
+          x[u].y = fmaf(c, t0[u].y, fmaf(e, t0[u].x, x[u].y));
         }
 #pragma unroll
-        for (int u = 0; u < GRP; u++) lds_at(acc, lc[u]) = x[u];
-        p = pn;
+        for (int u = 0; u < GRP; u++) lds_at(acc, l0[u]) = x[u];
         vx = vxn;
         vy = vyn;
-        k = kn;
-        s0 = sn;
+        cj = pn.conj;
+        kc = kn;
+        pn = pnn;
+        sn = snn;
+        kn = knn;
+        vxn = vxnn;
+        vyn = vynn;
+        more = kc < npw;
       };
-      // Both bodies run on every trip (the second one past the end works on
-      // the empty pair and touches only the dummy cell): a loop exit between
-      // them would join the back edge with undefined state, which costs
-      // vector copies on every group.
+      // The three bodies run on every trip (past the end they work on the
+      // empty pair and touch only the dummy cell): a loop exit between them
+      // would join the back edge with undefined state, which costs vector
+      // copies on every group.
       do {
-        body(ta, la, tb, lb);
+        body(ta, la, tc3, lc3);
         body(tb, lb, ta, la);
+        body(tc3, lc3, tb, lb);
       } while (more);
       // Never true (chunk >= 1): a use of both buffers on the loop exits
       // keeps the compiler from sinking the prefetch loads past the exit
@@ -641,8 +660,8 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
         float2 z = make_float2(0.f, 0.f);
 #pragma unroll
         for (int u = 0; u < GRP; u++) {
-          z.x += ta[u].x + tb[u].x;
-          z.y += ta[u].y + tb[u].y;
+          z.x += ta[u].x + tb[u].x + tc3[u].x;
+          z.y += ta[u].y + tb[u].y + tc3[u].y;
         }
         acc[wave * CP + lane] = z;
       }
