@@ -5,13 +5,17 @@
 // MS4/misc/cppcycle.cpp:210-265).
 //
 // Each call: gather the per-chunk uvw/vis pointers into one contiguous
-// buffer, find the w-plane range of the in-bound points on the GPU, upload
-// only the GCF layers of those planes, pregrid (FP64, exact), grid or degrid
-// with the FP64 kernels, copy the result back.  No state survives the call,
-// as in the reference; calls from different host threads use their own
-// streams and buffers.
+// buffer (or copy straight from the caller's memory when its chunks are
+// consecutive, as the reference's callers lay them out), find the w-plane
+// range of the in-bound points on the GPU, upload only the GCF layers of
+// those planes, pregrid (FP64, exact), grid or degrid with the FP64 kernels,
+// copy the result back.  No state survives the call, as in the reference;
+// calls from different host threads use their own streams and buffers.
 #include <climits>
 #include <cstring>
+#include <memory>
+#include <new>
+#include <type_traits>
 #include <vector>
 
 #include "rcg_internal.h"
@@ -68,6 +72,11 @@ ull legacy_call(const LegacyArgs &a)
     }
   };
   auto fail = [&]() -> ull { zero_outputs(); return 0ULL; };
+  // host staging without value-initialisation (the copies fill it)
+  auto host_array = [](size_t count, auto *tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    return std::unique_ptr<T[]>(new (std::nothrow) T[count ? count : 1]);
+  };
 
   if (a.baselines < 0 || a.ts_ch < 0 || a.size <= 0 || a.size > 32767 ||
       a.pitch < a.size || (long long)a.size * a.pitch > 0x7fffffffLL) {
@@ -80,33 +89,54 @@ ull legacy_call(const LegacyArgs &a)
     set_error(RCG_ERR_ARG, "null grid or visibility table");
     return 0;
   }
-  zero_outputs();
   const long long n = (long long)a.baselines * a.ts_ch;
-  if (n == 0) return 0;
+  if (n == 0) {
+    zero_outputs();
+    return 0;
+  }
   if (n > 0x7fffffffLL) {
     set_error(RCG_ERR_ARG, "%lld points exceed 2^31 - 1", n);
-    return 0;
+    return fail();
   }
   if (!a.bl_supps || !a.uvw || !a.gcf || !a.gcf_supps) {
     set_error(RCG_ERR_ARG, "null support, uvw or GCF table");
-    return 0;
+    return fail();
   }
   for (int bl = 0; bl < a.baselines; bl++) {
     if (a.bl_supps[bl] < 0 || a.bl_supps[bl] > MAX_SUPPORT) {
       set_error(RCG_ERR_ARG, "chunk %d support %d outside [0, %d]", bl,
                 a.bl_supps[bl], MAX_SUPPORT);
-      return 0;
+      return fail();
     }
     if (!a.uvw[bl] || (a.degrid ? !a.vis_out[bl] : !a.vis_in[bl])) {
       set_error(RCG_ERR_ARG, "chunk %d: null uvw or vis pointer", bl);
-      return 0;
+      return fail();
     }
   }
 
   // ---- gather host inputs --------------------------------------------------
-  std::vector<rcg_double3> huvw((size_t)n);
-  for (int bl = 0; bl < a.baselines; bl++)
-    memcpy(&huvw[(size_t)bl * a.ts_ch], a.uvw[bl], (size_t)a.ts_ch * sizeof(rcg_double3));
+  // Chunks laid out one after the other in one array are copied from there
+  // directly; otherwise they are gathered into a staging buffer.
+  auto consecutive = [&](const void *const *tab, size_t elem) {
+    const char *b0 = static_cast<const char *>(tab[0]);
+    for (int bl = 1; bl < a.baselines; bl++)
+      if (static_cast<const char *>(tab[bl]) != b0 + (size_t)bl * a.ts_ch * elem) return false;
+    return true;
+  };
+  const bool uvw_flat = consecutive(reinterpret_cast<const void *const *>(a.uvw),
+                                    sizeof(rcg_double3));
+  std::unique_ptr<rcg_double3[]> huvw_buf;
+  const rcg_double3 *huvw = a.uvw[0];
+  if (!uvw_flat) {
+    huvw_buf = host_array((size_t)n, (rcg_double3 *)nullptr);
+    if (!huvw_buf) {
+      set_error(RCG_ERR_NOMEM, "no host memory for %lld points", n);
+      return fail();
+    }
+    for (int bl = 0; bl < a.baselines; bl++)
+      memcpy(&huvw_buf[(size_t)bl * a.ts_ch], a.uvw[bl], (size_t)a.ts_ch * sizeof(rcg_double3));
+    huvw = huvw_buf.get();
+  }
 
   StreamGuard sg;
   LEG_TRY(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
@@ -119,7 +149,7 @@ ull legacy_call(const LegacyArgs &a)
   LEG_TRY(d_pg.alloc((size_t)n * sizeof(rcg_pg)));
   LEG_TRY(d_taps.alloc(sizeof(unsigned long long)));
   LEG_TRY(d_err.alloc(sizeof(int)));
-  LEG_TRY(hipMemcpyAsync(d_uvw.p, huvw.data(), (size_t)n * sizeof(rcg_double3),
+  LEG_TRY(hipMemcpyAsync(d_uvw.p, huvw, (size_t)n * sizeof(rcg_double3),
                          hipMemcpyHostToDevice, st));
   LEG_TRY(hipMemcpyAsync(d_supps.p, a.bl_supps, (size_t)a.baselines * sizeof(int),
                          hipMemcpyHostToDevice, st));
@@ -141,7 +171,10 @@ ull legacy_call(const LegacyArgs &a)
     return fail();
   LEG_TRY(hipMemcpyAsync(hrange, d_range.p, sizeof(hrange), hipMemcpyDeviceToHost, st));
   LEG_TRY(hipStreamSynchronize(st));
-  if (hrange[0] > hrange[1]) return 0;  // no in-bound point: all zero
+  if (hrange[0] > hrange[1]) {  // no in-bound point: all zero
+    zero_outputs();
+    return 0;
+  }
 
   // ---- upload the GCF layers in use ----------------------------------------
   // Supports are needed for the w-planes of the in-bound points, layers for
@@ -219,10 +252,20 @@ ull legacy_call(const LegacyArgs &a)
   LEG_TRY(d_vis.alloc((size_t)n * sizeof(rcg_c128)));
   LEG_TRY(d_grid.alloc(gcells * sizeof(rcg_c128)));
   if (!a.degrid) {
-    std::vector<rcg_c128> hvis((size_t)n);
-    for (int bl = 0; bl < a.baselines; bl++)
-      memcpy(&hvis[(size_t)bl * a.ts_ch], a.vis_in[bl], (size_t)a.ts_ch * sizeof(rcg_c128));
-    LEG_TRY(hipMemcpyAsync(d_vis.p, hvis.data(), (size_t)n * sizeof(rcg_c128),
+    std::unique_ptr<rcg_c128[]> hvis_buf;
+    const rcg_c128 *hvis = a.vis_in[0];
+    if (!consecutive(reinterpret_cast<const void *const *>(a.vis_in), sizeof(rcg_c128))) {
+      hvis_buf = host_array((size_t)n, (rcg_c128 *)nullptr);
+      if (!hvis_buf) {
+        set_error(RCG_ERR_NOMEM, "no host memory for %lld visibilities", n);
+        return fail();
+      }
+      for (int bl = 0; bl < a.baselines; bl++)
+        memcpy(&hvis_buf[(size_t)bl * a.ts_ch], a.vis_in[bl],
+               (size_t)a.ts_ch * sizeof(rcg_c128));
+      hvis = hvis_buf.get();
+    }
+    LEG_TRY(hipMemcpyAsync(d_vis.p, hvis, (size_t)n * sizeof(rcg_c128),
                            hipMemcpyHostToDevice, st));
     if (rcg_grid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_vis.as<rcg_c128>(),
                            d_grid.as<rcg_c128>(), st) != RCG_OK)
@@ -236,12 +279,22 @@ ull legacy_call(const LegacyArgs &a)
     if (rcg_degrid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_grid.as<rcg_c128>(),
                              d_vis.as<rcg_c128>(), st) != RCG_OK)
       return fail();
-    std::vector<rcg_c128> hvis((size_t)n);
-    LEG_TRY(hipMemcpyAsync(hvis.data(), d_vis.p, (size_t)n * sizeof(rcg_c128),
-                           hipMemcpyDeviceToHost, st));
-    LEG_TRY(hipStreamSynchronize(st));
-    for (int bl = 0; bl < a.baselines; bl++)
-      memcpy(a.vis_out[bl], &hvis[(size_t)bl * a.ts_ch], (size_t)a.ts_ch * sizeof(rcg_c128));
+    if (consecutive(reinterpret_cast<const void *const *>(a.vis_out), sizeof(rcg_c128))) {
+      LEG_TRY(hipMemcpyAsync(a.vis_out[0], d_vis.p, (size_t)n * sizeof(rcg_c128),
+                             hipMemcpyDeviceToHost, st));
+      LEG_TRY(hipStreamSynchronize(st));
+    } else {
+      std::unique_ptr<rcg_c128[]> hvis = host_array((size_t)n, (rcg_c128 *)nullptr);
+      if (!hvis) {
+        set_error(RCG_ERR_NOMEM, "no host memory for %lld visibilities", n);
+        return fail();
+      }
+      LEG_TRY(hipMemcpyAsync(hvis.get(), d_vis.p, (size_t)n * sizeof(rcg_c128),
+                             hipMemcpyDeviceToHost, st));
+      LEG_TRY(hipStreamSynchronize(st));
+      for (int bl = 0; bl < a.baselines; bl++)
+        memcpy(a.vis_out[bl], &hvis[(size_t)bl * a.ts_ch], (size_t)a.ts_ch * sizeof(rcg_c128));
+    }
   }
   unsigned long long taps = 0;
   int err = 0;
