@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision", "binner",
-                                           "flow", "small", "next"],
+                                           "flow", "small", "next", "ms4"],
                     default="wproj",
                     help="wproj: the 4096^2 w-projection workload (default); batched: 64 "
                          "uneven 2048^2 snapshot instances, one graph launch vs one launch "
@@ -55,7 +55,9 @@ def parse():
                          "MS4 uv binner (binm) through its host-memory C ABI; flow: the "
                          "MS6 binSizer + binner kernels; small: normalize, rotate, grid0, "
                          "reweight, hermitian fold and FFT against their rooflines; next: "
-                         "GCF generation, MS6 scatter/degrid and nvGridder")
+                         "GCF generation, MS6 scatter/degrid and nvGridder; ms4: the MS4 "
+                         "host-memory grid/degrid calls on 32640 chunks of 306 and the "
+                         "device FP64 gridders")
     ap.add_argument("--instances", type=int, default=64)
     return ap.parse_args()
 
@@ -463,6 +465,128 @@ def bench_binner(args) -> int:
     return 0
 
 
+def bench_ms4(args) -> int:
+    """The MS4 host-memory entry points (gridKernelCPUHalfGCF +
+    deGridKernelCPUHalfGCF, MS4/kernel/cpu/scatter_grid/
+    scatter_gridder_w_dependent_dyn_1p.cpp:245-266, 368-431) on the
+    w-projection points in the reference's chunk layout: 32640 baselines x
+    306 samples, each chunk's support that of its smallest-|w| plane,
+    complex128 throughout.  One step = one grid call + one degrid call of the
+    gridded result, host memory in and out (PCIe copies included).  Also
+    reported: the device-resident FP64 gridders on the same points, timed
+    with HIP events on the stream they run on: rcg_grid_tiled_f64 (binned
+    pairs, what the grid call runs) and the plain FP64 gridder."""
+    import torch
+    from rc_amd import device as D
+    from rc_amd import ms4, synth
+    from rc_amd.tables import VisChunks
+
+    vc, gcf, N = synth.wproj(n_times=args.n_times)
+    ts = args.n_times
+    nb = vc.n // ts
+    uvw = vc.uvw.reshape(nb, ts, 3)
+    vis = vc.vis.reshape(nb, ts).astype(np.complex128)
+    sup = vc.supps.reshape(nb, ts).min(axis=1).astype(np.int32)
+    v2 = VisChunks(uvw=uvw, vis=vis, supps=sup, scale=vc.scale, wstep=vc.wstep)
+    g128 = gcf.astype(np.complex128)
+    n = vc.n
+
+    # device-resident FP64 gridders
+    stream = torch.cuda.current_stream()
+    geom = D.DeviceGeom.from_chunks(v2, N)
+    dg = D.DeviceGCF.from_table(g128)
+    dvis = torch.from_numpy(vis.reshape(-1)).cuda()
+    pg, taps_d, _ = D.pregrid(geom, dg)
+    plan = D.TiledPlan(geom, tile=32, chunk=1024)
+    plan.bin(dg)
+    gt = plan.grid_f64(dg, dvis)
+    gp = D.grid_plain_f64(geom, dg, pg, dvis)
+    torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    ms_tiled = timed(lambda: plan.grid_f64(dg, dvis, gt), max(3, args.steps))
+    ms_plain = timed(lambda: D.grid_plain_f64(geom, dg, pg, dvis, gp), 3)
+    rel = float((gt - gp).abs().pow(2).sum().sqrt() / gp.abs().pow(2).sum().sqrt())
+    taps = int(taps_d.item())
+    plan.close()
+    del gt, gp, pg, dvis, dg, geom
+    torch.cuda.empty_cache()
+
+    # the host-memory calls
+    def step():
+        t0 = time.perf_counter()
+        g, tg = ms4.grid(v2, g128, N)
+        t1 = time.perf_counter()
+        g /= N * N
+        t2 = time.perf_counter()
+        _, td = ms4.degrid(v2, g128, g)
+        t3 = time.perf_counter()
+        assert tg == td == taps
+        return t1 - t0, t3 - t2
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    ts_g, ts_d = zip(*[step() for _ in range(args.steps)])
+    mg, md = float(np.median(ts_g)) * 1e3, float(np.median(ts_d)) * 1e3
+    ms = mg + md
+    pcie = (24 + 16) * n * 2 + 16 * N * N * 2  # uvw twice, vis in + out, grid out + in
+    line = {
+        "metric": "Mvis/s gridded+degridded through the MS4 host-memory calls "
+                  "(gridKernelCPUHalfGCF + deGridKernelCPUHalfGCF, PCIe copies included)",
+        "value": round(n / (ms / 1e3) / 1e6, 3), "unit": "Mvis/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128/f64", "data": "synthetic",
+        "config": {"workload": "w-projection points in 32640 chunks of 306 (256 antennas), "
+                               "4096x4096, 32 w-planes, chunk supports 8-64, 8x oversampling",
+                   "points": n, "taps_per_direction": taps},
+        "calls_ms": {"grid": round(mg, 3), "degrid": round(md, 3)},
+        "host_copy_bytes_per_step": pcie,
+        "device_fp64_grid_ms": {"tiled": round(ms_tiled, 4), "plain": round(ms_plain, 4),
+                                "speedup": round(ms_plain / ms_tiled, 2),
+                                "tiled_gtaps_per_s": round(taps / ms_tiled / 1e6, 1),
+                                "rel_l2_tiled_vs_plain": rel},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(16, len(os.sched_getaffinity(0)))
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        kind = "reference" if O.ref_available(fast=True) else "port"
+        impl = "ref_fast" if kind == "reference" else "oracle"
+
+        def run(m):
+            s = VisChunks(uvw=uvw[:m], vis=vis[:m], supps=sup[:m], scale=vc.scale,
+                          wstep=vc.wstep)
+            t0 = time.perf_counter()
+            g, tp = O.grid(s, g128, N, impl=impl, nthreads=threads)
+            t1 = time.perf_counter()
+            O.degrid(s, g128, g, impl=impl, nthreads=threads)
+            t2 = time.perf_counter()
+            return t2 - t0, t1 - t0, t2 - t1, tp
+
+        m = 300
+        dt, _, _, _ = run(m)
+        m2 = int(min(nb, max(m, m * args.cpu_seconds / max(dt, 1e-3))))
+        dt, tg, td, tp = run(m2)
+        line["cpu_baseline"] = {
+            "value": round(m2 * ts / dt / 1e6, 4), "unit": "Mvis/s", "cores": threads,
+            "kind": kind,
+            "sample": f"first {m2} chunks ({m2 * ts} points, taps {tp}), full 4096^2 "
+                      f"complex128 grid, grid {tg:.2f} s + degrid {td:.2f} s, "
+                      f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_flow(args) -> int:
     """The MS6 visibility binning kernels (binSizer + binner) through their
     host-memory C ABI on the points of the w-projection workload as 5-double
@@ -788,6 +912,8 @@ def main() -> int:
         return bench_small(args)
     if args.workload == "next":
         return bench_next(args)
+    if args.workload == "ms4":
+        return bench_ms4(args)
     dev = torch.device("cuda:0")
     t0 = time.time()
     vc, gcf, N = synth.wproj(n_times=args.n_times)

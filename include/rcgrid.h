@@ -235,7 +235,7 @@ typedef struct rcg_pg {
   int32_t flags;    /* bit 0: conjugate taps; bit 1: range error       */
 } rcg_pg;
 
-/* FP64 exact pregrid of every point (reference pregridPoint + SG:128-144).
+/* FP64 exact pregrid of every point (reference pregridPoint + scatter_gridder_w_dependent_dyn_1p.cpp:128-144).
  * pg must hold n_chunks*ts_ch records.  *taps_dev (device, optional)
  * receives the in-bound tap count.  Status RCG_ERR_RANGE is reported by
  * rcg_plain_check() after the stream is synchronised. */
@@ -288,9 +288,17 @@ rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
 rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                                 const rcg_c64 *grid, rcg_c64 *vis,
                                 void *stream);
+/* FP64 gridding over a plan's pairs (after rcg_tiled_bin with an FP64
+ * tap table, the same GCF layout): each tap adds the reference's complex
+ * product in FP64 (scatter_gridder_w_dependent_dyn_1p.cpp:169, the body of
+ * gridKernel_scatter, :54-186, behind gridKernelCPUFullGCF / HalfGCF); only the summation order
+ * differs from the sequential one.  grid is zeroed first.  vis/grid
+ * rcg_c128.  The gridder of the MS4 host-memory entry points. */
+rcg_status rcg_grid_tiled_f64(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                              const rcg_c128 *vis, rcg_c128 *grid, void *stream);
 /* The Hermitian adjoint of rcg_degrid_tiled_f32: gridding with conjugated
  * taps (the reference's degridder is the plain transpose of its gridder,
- * SG:368-431, so the adjoint conjugates).  Used as the backward pass of a
+ * scatter_gridder_w_dependent_dyn_1p.cpp:368-431, so the adjoint conjugates).  Used as the backward pass of a
  * measurement operator. */
 rcg_status rcg_grid_tiled_adjoint_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                                       const rcg_c64 *vis, rcg_c64 *grid,

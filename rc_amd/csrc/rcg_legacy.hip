@@ -8,9 +8,11 @@
 // buffer (or copy straight from the caller's memory when its chunks are
 // consecutive, as the reference's callers lay them out), find the w-plane
 // range of the in-bound points on the GPU, upload only the GCF layers of
-// those planes, pregrid (FP64, exact), grid or degrid with the FP64 kernels,
-// copy the result back.  No state survives the call, as in the reference;
+// those planes, pregrid (FP64, exact), grid (tiled FP64 gridder over binned
+// pairs, or the plain FP64 gridder where no plan fits) or degrid (plain FP64
+// degridder), copy the result back.  No state survives the call, as in the reference;
 // calls from different host threads use their own streams and buffers.
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <memory>
@@ -19,6 +21,7 @@
 #include <vector>
 
 #include "rcg_internal.h"
+#include "rcg_tiled.h"
 
 namespace rcg {
 namespace {
@@ -28,6 +31,11 @@ struct DevBuf {
   ~DevBuf() { if (p) (void)hipFree(p); }
   hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
   template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct PlanGuard {
+  rcg_tiled_plan *p = nullptr;
+  ~PlanGuard() { if (p) (void)rcg_tiled_plan_destroy(p); }
 };
 
 struct StreamGuard {
@@ -267,9 +275,34 @@ ull legacy_call(const LegacyArgs &a)
     }
     LEG_TRY(hipMemcpyAsync(d_vis.p, hvis, (size_t)n * sizeof(rcg_c128),
                            hipMemcpyHostToDevice, st));
-    if (rcg_grid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_vis.as<rcg_c128>(),
-                           d_grid.as<rcg_c128>(), st) != RCG_OK)
+    // The tiled FP64 gridder over binned pairs where a plan fits the grid
+    // (at most 16384 tiles) and the supports (at most 64 tiles per window);
+    // otherwise the plain FP64 gridder.
+    PlanGuard plan;
+    int max_s = 1;
+    for (int bl = 0; bl < a.baselines; bl++) max_s = std::max(max_s, a.bl_supps[bl]);
+    for (int tile : {32, 64}) {
+      const long long nt = (a.size + tile - 1) / tile;
+      if (nt * nt > 16384) continue;
+      if (rcg_tiled_plan_create(&g, tile, 1024, max_s, &plan.p) == RCG_OK) break;
+      plan.p = nullptr;
+    }
+    clear_error();
+    if (plan.p) {
+      if (rcg_tiled_bin(plan.p, &t, st) != RCG_OK ||
+          rcg_grid_tiled_f64(plan.p, &t, d_vis.as<rcg_c128>(), d_grid.as<rcg_c128>(), st) !=
+              RCG_OK)
+        return fail();
+      int perr = 0;
+      if (rcg_tiled_stats(plan.p, nullptr, nullptr, nullptr, &perr, st) != RCG_OK) return fail();
+      if (perr & ERR_CAPACITY) {
+        set_error(RCG_ERR_HIP, "gridding plan: pair buffer overflow");
+        return fail();
+      }
+    } else if (rcg_grid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_vis.as<rcg_c128>(),
+                                  d_grid.as<rcg_c128>(), st) != RCG_OK) {
       return fail();
+    }
     LEG_TRY(hipMemcpyAsync(a.grid_out, d_grid.p, gcells * sizeof(rcg_c128),
                            hipMemcpyDeviceToHost, st));
     LEG_TRY(hipStreamSynchronize(st));

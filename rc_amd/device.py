@@ -215,6 +215,28 @@ class TiledPlan:
                    "rcg_grid_tiled_f32")
         return grid
 
+    def grid_f64(self, gcf: DeviceGCF, vis: torch.Tensor, grid: torch.Tensor | None = None,
+                 stream=None) -> torch.Tensor:
+        """FP64 gridding over the plan's pairs (rcg_grid_tiled_f64): complex128
+        taps (the plan binned with a table of the same layout), visibilities
+        and grid."""
+        if gcf.data.dtype != torch.complex128:
+            raise TypeError("FP64 gridding takes complex128 taps")
+        if vis.dtype != torch.complex128:
+            raise TypeError("FP64 gridding takes complex128 visibilities")
+        if vis.numel() != self.n or not vis.is_contiguous():
+            raise ValueError("vis must be a contiguous tensor of n points")
+        if grid is None:
+            grid = torch.empty(self.grid_shape, dtype=torch.complex128, device=vis.device)
+        if grid.dtype != torch.complex128 or tuple(grid.shape) != self.grid_shape or \
+                not grid.is_contiguous():
+            raise ValueError(f"grid must be a contiguous complex128 {self.grid_shape} tensor")
+        t = gcf.struct()
+        _lib.check(_lib.lib().rcg_grid_tiled_f64(self._h, C.byref(t), vis.data_ptr(),
+                                                 grid.data_ptr(), _stream(stream)),
+                   "rcg_grid_tiled_f64")
+        return grid
+
     def degrid(self, gcf: DeviceGCF, grid: torch.Tensor, vis: torch.Tensor | None = None,
                stream=None) -> torch.Tensor:
         if gcf.data.dtype != torch.complex64:

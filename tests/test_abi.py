@@ -3,6 +3,7 @@ include/ declare (no compute call; runs without a GPU)."""
 import ctypes
 import re
 import subprocess
+import threading
 
 import pytest
 
@@ -43,4 +44,10 @@ def test_library_loads_without_a_gpu():
     ctypes.CDLL(str(LIB))
     from rc_amd import _lib
     h = _lib.lib()
-    assert h.rcg_last_error() == 0
+    # the error state is per thread: a thread that made no call has none,
+    # whatever earlier tests left on this one
+    seen = []
+    t = threading.Thread(target=lambda: seen.append(h.rcg_last_error()))
+    t.start()
+    t.join()
+    assert seen == [0]
