@@ -223,17 +223,23 @@ ull legacy_call(const LegacyArgs &a)
     hlen[(size_t)s.slot] = s.len;
     total += s.len;
   }
-  std::vector<rcg_c128> hdata((size_t)(total > 0 ? total : 1));
+  const size_t n_data = (size_t)(total > 0 ? total : 1);
+  std::unique_ptr<rcg_c128[]> hdata = host_array(n_data, (rcg_c128 *)nullptr);
+  if (!hdata) {
+    set_error(RCG_ERR_NOMEM, "no host memory for %lld GCF taps", total);
+    return fail();
+  }
+  if (total == 0) hdata[0] = rcg_c128{0.0, 0.0};
   for (const Src &s : srcs)
     if (s.ptr && s.len)
       memcpy(&hdata[(size_t)hoff[(size_t)s.slot]], s.ptr, (size_t)s.len * sizeof(rcg_c128));
 
   DevBuf d_data, d_off, d_len, d_tsupp;
-  LEG_TRY(d_data.alloc(hdata.size() * sizeof(rcg_c128)));
+  LEG_TRY(d_data.alloc(n_data * sizeof(rcg_c128)));
   LEG_TRY(d_off.alloc(hoff.size() * sizeof(int64_t)));
   LEG_TRY(d_len.alloc(hlen.size() * sizeof(int32_t)));
   LEG_TRY(d_tsupp.alloc(hsupp.size() * sizeof(int32_t)));
-  LEG_TRY(hipMemcpyAsync(d_data.p, hdata.data(), hdata.size() * sizeof(rcg_c128),
+  LEG_TRY(hipMemcpyAsync(d_data.p, hdata.get(), n_data * sizeof(rcg_c128),
                          hipMemcpyHostToDevice, st));
   LEG_TRY(hipMemcpyAsync(d_off.p, hoff.data(), hoff.size() * sizeof(int64_t),
                          hipMemcpyHostToDevice, st));
@@ -250,7 +256,7 @@ ull legacy_call(const LegacyArgs &a)
   t.supps = d_tsupp.as<int32_t>();
   t.max_w_plane = M;
   t.half = a.half ? 1 : 0;
-  t.n_taps = (int64_t)hdata.size();
+  t.n_taps = (int64_t)n_data;
 
   if (rcg_pregrid(&g, &t, d_pg.as<rcg_pg>(), d_taps.as<unsigned long long>(),
                   d_err.as<int>(), st) != RCG_OK)
