@@ -85,6 +85,18 @@ def test_support_shapes_vs_oracle(hip_device, supports, half, pitch_pad):
     assert rel_l2(vis, vref) <= TOL
 
 
+def test_supports_above_225_bin_at_tile_64(hip_device):
+    """Chunk supports above 225 span more than 64 tiles of 32, so the grid
+    call bins its plan at T = 64 instead (supports 232 and 256, the largest
+    the path accepts)."""
+    vc, gcf, N = synth.random_small(22, grid_size=600, chunks=6, ts_ch=7, max_plane=1,
+                                    half=True, supports=(232, 256), chunk_supp="min")
+    g, taps = ms4.grid(vc, gcf, N)
+    ref, rtaps = O.grid(vc, gcf, N)
+    assert taps == rtaps > 0
+    assert rel_l2(g, ref) <= TOL
+
+
 def test_empty_and_all_out_of_bound(hip_device):
     vc, gcf, N = synth.random_small(3)
     empty = VisChunks(uvw=np.zeros((0, 5, 3)), vis=np.zeros((0, 5), np.complex128),
