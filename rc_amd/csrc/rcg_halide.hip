@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "rcg_internal.h"
+#include "rcg_tiled.h"
 #include "../../include/rcgrid_halide.h"
 
 namespace rcg {
@@ -99,6 +100,35 @@ k_h_scatter(const double *__restrict__ vis, HB vb, const double *__restrict__ gc
     unsafeAtomicAdd(c, pr);
     unsafeAtomicAdd(c + ub.str[0], pi);
   }
+}
+
+// Records of the tiled FP64 gridder for kern_scatter on dense buffers: per
+// point the window's first grid cell (rows = y, columns = x, cell = (y0 -
+// min_y) * pitch + (x0 - min_x)) and the first tap of its oversampling
+// layer (tap (x, y) at over * ostr + y * tstr + x complex elements), or cell
+// -1 for a point k_h_scatter would skip; and the visibility packed as a
+// complex double.
+__global__ void __launch_bounds__(256)
+k_h_records(const double *__restrict__ vis, HB vb, HB ub, double scale, int grid_size,
+            int margin, int S, int pitch, int tstr, long long ostr, rcg_pg *__restrict__ pg,
+            double2 *__restrict__ v2, long long n)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *r = vis + i * vb.str[1];
+  const Pix p = h_pixel(r[0], r[vb.str[0]], scale, grid_size, S, ub, margin);
+  rcg_pg rec;
+  rec.cell = -1;
+  rec.tap0 = 0;
+  rec.stride = (int16_t)tstr;
+  rec.supp = (int16_t)S;
+  rec.flags = 0;
+  if (p.ok) {
+    rec.cell = (p.y0 - ub.mn[2]) * pitch + (p.x0 - ub.mn[1]);
+    rec.tap0 = (int)(p.over * ostr);
+  }
+  pg[i] = rec;
+  v2[i] = make_double2(r[3 * vb.str[0]], r[4 * vb.str[0]]);
 }
 
 __global__ void __launch_bounds__(256)
@@ -217,6 +247,80 @@ void drop_device(buffer_t *b)
   b->host_dirty = false;
 }
 
+struct PlanHold {
+  rcg_tiled_plan *p = nullptr;
+  double2 *v2 = nullptr;
+  ~PlanHold()
+  {
+    if (v2) (void)hipFree(v2);
+    if (p) (void)rcg_tiled_plan_destroy(p);
+  }
+};
+
+// kern_scatter through the tiled FP64 gridder: binned pairs, one FP64 tile
+// per workgroup in LDS, tile sums added to the grid.  Taken for a square
+// grid window (extent x = extent y) with interleaved complex doubles (re/im
+// stride 1, x stride 2, even y stride) and a GCF of the same kind, where a
+// plan fits (at most 16384 tiles of 32 or 64, 2^31 pairs).  Returns 1 when
+// the points were gridded, 0 when the caller should run k_h_scatter, -1 on
+// a failure after work was queued (error set).
+int scatter_tiled(double scale, int grid_size, int margin, const buffer_t *vis,
+                  const buffer_t *gcf, const buffer_t *uvg, const DevData *d, int S, long long n,
+                  hipStream_t st)
+{
+  const int X = uvg->extent[1];
+  if (X != uvg->extent[2] || uvg->stride[0] != 1 || uvg->stride[1] != 2 ||
+      (uvg->stride[2] & 1) || gcf->stride[0] != 1 || gcf->stride[1] != 2 ||
+      (gcf->stride[2] & 1) || (gcf->stride[3] & 1) || gcf->stride[2] / 2 > 32767 ||
+      span_of(gcf) >= 0x7fffffffLL || n > INT_MAX)
+    return 0;
+  const int pitch = uvg->stride[2] / 2;
+  // The plan's geometry: n points in chunks of one on the X x X window.  Its
+  // uvw and chunk supports are read only by the pregrid of rcg_tiled_bin,
+  // which this route replaces with k_h_records; they point at the
+  // visibility records to pass the plan's argument checks.
+  rcg_geom g{};
+  g.uvw = reinterpret_cast<const rcg_double3 *>(d[0].p);
+  g.chunk_supps = reinterpret_cast<const int32_t *>(d[0].p);
+  g.n_chunks = (int)n;
+  g.ts_ch = 1;
+  g.scale = scale;
+  g.wstep = 1.0;
+  g.grid_size = X;
+  g.grid_pitch = pitch;
+  PlanHold h;
+  for (int tile : {32, 64}) {
+    const long long nt = (X + tile - 1) / tile;
+    if (nt * nt > 16384) continue;
+    if (rcg_tiled_plan_create(&g, tile, 1024, S, &h.p) == RCG_OK) break;
+    h.p = nullptr;
+  }
+  clear_error();
+  if (!h.p) return 0;
+  if (hipMalloc(reinterpret_cast<void **>(&h.v2), (size_t)n * sizeof(double2)) != hipSuccess) {
+    h.v2 = nullptr;
+    return 0;
+  }
+  hipLaunchKernelGGL(k_h_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d[0].p,
+                     hb_of(vis), hb_of(uvg), scale, grid_size, margin, S, pitch,
+                     gcf->stride[2] / 2, (long long)(gcf->stride[3] / 2), h.p->pg, h.v2, n);
+  if (hipGetLastError() != hipSuccess) {
+    set_error(RCG_ERR_HIP, "kern_scatter: record kernel launch failed");
+    return -1;
+  }
+  if (tiled_bin_records(h.p, st) != RCG_OK ||
+      tiled_grid_f64_add(h.p, reinterpret_cast<const double2 *>(d[1].p), span_of(gcf) / 2, h.v2,
+                         reinterpret_cast<double2 *>(d[2].p), st) != RCG_OK)
+    return -1;
+  int err = 0;
+  if (rcg_tiled_stats(h.p, nullptr, nullptr, nullptr, &err, st) != RCG_OK) return -1;
+  if (err) {
+    set_error(RCG_ERR_HIP, "kern_scatter: gridding plan error %d", err);
+    return -1;
+  }
+  return 1;
+}
+
 int scatter(double scale, int grid_size, int margin, buffer_t *vis, buffer_t *gcf, buffer_t *uvg,
             bool gpu_mode, int bad_size)
 {
@@ -240,16 +344,20 @@ int scatter(double scale, int grid_size, int margin, buffer_t *vis, buffer_t *gc
     rc = -1;
   } else {
     const long long n = vis->extent[1];
-    if (n > 0) {
+    int tiled = 0;
+    if (n > 0) tiled = scatter_tiled(scale, grid_size, margin, vis, gcf, uvg, d, S, n, st);
+    if (tiled < 0) {
+      rc = -1;
+    } else if (n > 0 && !tiled) {
       const long long blocks = (n * 64 + 255) / 256;
       hipLaunchKernelGGL(k_h_scatter, dim3((unsigned)blocks), dim3(256), 0, st, d[0].p,
                          hb_of(vis), d[1].p, hb_of(gcf), d[2].p, hb_of(uvg), scale, grid_size,
                          margin, S);
     }
     const size_t bytes = (size_t)span_of(uvg) * sizeof(double);
-    if (hipGetLastError() != hipSuccess ||
+    if (rc == 0 && (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(uvg->host, d[2].p, bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+        hipStreamSynchronize(st) != hipSuccess)) {
       set_error(RCG_ERR_HIP, "%s: kernel or copy failed", who);
       rc = -1;
     }

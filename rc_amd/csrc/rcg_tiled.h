@@ -51,6 +51,21 @@ constexpr int N_TIMED = 64;
 
 }  // namespace rcg
 
+struct rcg_tiled_plan;
+
+namespace rcg {
+
+// Bins the records the caller wrote into the plan's record buffer
+// (rcg_tiled_plan::pg, one per point; cell -1 for a point left out) in place
+// of the pregrid that rcg_tiled_bin runs.
+rcg_status tiled_bin_records(rcg_tiled_plan *pl, hipStream_t st);
+
+// FP64 gridding over the plan's pairs, added to the grid's contents.
+rcg_status tiled_grid_f64_add(rcg_tiled_plan *pl, const double2 *taps, long long n_taps,
+                              const double2 *vis, double2 *grid, hipStream_t st);
+
+}  // namespace rcg
+
 struct rcg_tiled_plan {
   std::vector<rcg_geom> geoms;
   rcg_geom *geoms_dev = nullptr;

@@ -715,7 +715,9 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
 // 4096^2 w-projection points in 32640 chunks of 306 (1.35 G taps): 2.9 ms,
 // against 8.6 ms with a private copy per wavefront (two workgroups per CU)
 // and 3.0-4.9 ms with the taps of 2-8 slabs loaded together.  A tap index
-// past the table reads nothing and adds nothing.
+// past the table reads nothing and adds nothing.  add: the tile sums are
+// added to the grid's contents (otherwise the grid was zeroed and a tile
+// with a single work item is stored).
 
 constexpr int F64_WAVES = 4;  // 2: 3.7 ms, 8: 3.2-3.5 ms on the workload above
 
@@ -723,7 +725,7 @@ template <int T>
 __global__ void __launch_bounds__(64 * F64_WAVES)
 k_grid_tiles_f64(TiledDev d, const double2 *__restrict__ vis,
                  const double2 *__restrict__ taps, long long n_taps,
-                 double2 *__restrict__ grid)
+                 double2 *__restrict__ grid, int add)
 {
 #pragma clang fp contract(off)
   constexpr int LS = T + 1;
@@ -768,7 +770,14 @@ k_grid_tiles_f64(TiledDev d, const double2 *__restrict__ vis,
       const double2 a = acc[r * LS + cc];
       double2 *g = gi + (long long)gu * d.pitch + gv;
       if (ir.single) {
-        *g = a;
+        if (add) {
+          double2 o = *g;
+          o.x += a.x;
+          o.y += a.y;
+          *g = o;
+        } else {
+          *g = a;
+        }
       } else if (a.x != 0.0 || a.y != 0.0) {
         unsafeAtomicAdd(&g->x, a.x);
         unsafeAtomicAdd(&g->y, a.y);
@@ -1251,6 +1260,10 @@ extern "C" rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *pl)
   return RCG_OK;
 }
 
+namespace {
+rcg_status bin_passes(rcg_tiled_plan *pl, hipStream_t st);
+}  // namespace
+
 extern "C" rcg_status rcg_tiled_bin(rcg_tiled_plan *pl, const rcg_gcf *gcf, void *stream)
 {
   clear_error();
@@ -1273,6 +1286,30 @@ extern "C" rcg_status rcg_tiled_bin(rcg_tiled_plan *pl, const rcg_gcf *gcf, void
   s = launch_pregrid_batch(pl->geoms_dev, d.pt_off, d.n_inst, pl->n, gcf, pl->pg, pl->taps,
                            pl->err, st);
   if (s != RCG_OK) return s;
+  return bin_passes(pl, st);
+}
+
+rcg_status rcg::tiled_bin_records(rcg_tiled_plan *pl, hipStream_t st)
+{
+  TiledDev &d = pl->dev;
+  hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, d.tile_count, 0,
+                     (long long)d.n_tiles);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.totals, 0, 4LL);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st,
+                     reinterpret_cast<int *>(pl->taps), 0, 2LL);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, pl->err, 0, 1LL);
+  RCG_HIP_TRY(hipGetLastError());
+  if (pl->n == 0) return RCG_OK;
+  return bin_passes(pl, st);
+}
+
+namespace {
+
+// Counting sort of the plan's records into per-tile pair lists: LDS
+// histograms, three-kernel scan, LDS-ranked fill.
+rcg_status bin_passes(rcg_tiled_plan *pl, hipStream_t st)
+{
+  TiledDev &d = pl->dev;
   const size_t lds_count = (size_t)d.tpi * sizeof(int);
   const size_t lds_fill =
       lds_count + (size_t)BIN_THREADS * d.ppt * d.mpp * sizeof(uint16_t);
@@ -1287,6 +1324,8 @@ extern "C" rcg_status rcg_tiled_bin(rcg_tiled_plan *pl, const rcg_gcf *gcf, void
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
+
+}  // namespace
 
 namespace {
 
@@ -1421,13 +1460,42 @@ namespace {
 
 template <int T>
 rcg_status launch_grid_f64(const TiledDev &d, const double2 *v, const double2 *t, long long nt,
-                           double2 *g, hipStream_t st)
+                           double2 *g, int add, hipStream_t st)
 {
   int nb = 0;
   RCG_HIP_TRY(persistent_blocks(reinterpret_cast<const void *>(&k_grid_tiles_f64<T>),
                                 64 * F64_WAVES, &nb));
-  hipLaunchKernelGGL(k_grid_tiles_f64<T>, dim3(nb), dim3(64 * F64_WAVES), 0, st, d, v, t, nt, g);
+  hipLaunchKernelGGL(k_grid_tiles_f64<T>, dim3(nb), dim3(64 * F64_WAVES), 0, st, d, v, t, nt, g,
+                     add);
   RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+rcg_status grid_f64(rcg_tiled_plan *pl, const double2 *v, const double2 *t, long long nt,
+                    double2 *g, int add, hipStream_t st)
+{
+  TiledDev d = pl->dev;
+  d.adj = 0;
+  d.lo = nullptr;
+  if (!add)
+    hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st, reinterpret_cast<float *>(g),
+                       d.gstride * d.n_inst * 4);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue, 0, 8LL);
+  RCG_HIP_TRY(hipGetLastError());
+  if (pl->n == 0) return RCG_OK;
+  hipEvent_t *ev = timing_slot(pl, 0);
+  if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
+  rcg_status s;
+  switch (d.T) {
+    case 16: s = launch_grid_f64<16>(d, v, t, nt, g, add, st); break;
+    case 32: s = launch_grid_f64<32>(d, v, t, nt, g, add, st); break;
+    default: s = launch_grid_f64<64>(d, v, t, nt, g, add, st); break;
+  }
+  if (s != RCG_OK) return s;
+  if (ev) {
+    RCG_HIP_TRY(hipEventRecord(ev[1], st));
+    pl->n_ev[0]++;
+  }
   return RCG_OK;
 }
 
@@ -1447,31 +1515,19 @@ extern "C" rcg_status rcg_grid_tiled_f64(rcg_tiled_plan *pl, const rcg_gcf *gcf,
     set_error(RCG_ERR_ARG, "rcg_grid_tiled_f64: empty tap table");
     return RCG_ERR_ARG;
   }
-  hipStream_t st = (hipStream_t)stream;
-  TiledDev d = pl->dev;
-  d.adj = 0;
-  d.lo = nullptr;
-  hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st,
-                     reinterpret_cast<float *>(grid), d.gstride * d.n_inst * 4);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue, 0, 8LL);
-  RCG_HIP_TRY(hipGetLastError());
-  if (pl->n == 0) return RCG_OK;
-  const double2 *v = reinterpret_cast<const double2 *>(vis);
-  const double2 *t = reinterpret_cast<const double2 *>(gcf->data);
-  double2 *g = reinterpret_cast<double2 *>(grid);
-  hipEvent_t *ev = timing_slot(pl, 0);
-  if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
-  switch (d.T) {
-    case 16: s = launch_grid_f64<16>(d, v, t, gcf->n_taps, g, st); break;
-    case 32: s = launch_grid_f64<32>(d, v, t, gcf->n_taps, g, st); break;
-    default: s = launch_grid_f64<64>(d, v, t, gcf->n_taps, g, st); break;
+  return grid_f64(pl, reinterpret_cast<const double2 *>(vis),
+                  reinterpret_cast<const double2 *>(gcf->data), gcf->n_taps,
+                  reinterpret_cast<double2 *>(grid), 0, (hipStream_t)stream);
+}
+
+rcg_status rcg::tiled_grid_f64_add(rcg_tiled_plan *pl, const double2 *taps, long long n_taps,
+                                   const double2 *vis, double2 *grid, hipStream_t st)
+{
+  if (n_taps <= 0) {
+    set_error(RCG_ERR_ARG, "tiled FP64 gridding: empty tap table");
+    return RCG_ERR_ARG;
   }
-  if (s != RCG_OK) return s;
-  if (ev) {
-    RCG_HIP_TRY(hipEventRecord(ev[1], st));
-    pl->n_ev[0]++;
-  }
-  return RCG_OK;
+  return grid_f64(pl, vis, taps, n_taps, grid, 1, st);
 }
 
 extern "C" rcg_status rcg_tiled_stats(rcg_tiled_plan *pl, unsigned long long *taps,

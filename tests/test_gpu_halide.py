@@ -42,6 +42,26 @@ def test_scatter_vs_oracle(hip_device, S, gpu1):
     assert np.count_nonzero(exp[..., 1]) > 0
 
 
+@pytest.mark.parametrize("S", [8, 16, 32, 64])
+@pytest.mark.parametrize("gpu1", [False, True])
+def test_scatter_square_window_vs_oracle(hip_device, S, gpu1):
+    """A square 224 x 224 window of the grid at (16, 24), row pitch 230: the
+    tiled FP64 route (points binned by 32 x 32 tile, tile sums added to the
+    grid).  The pad columns past the window stay as they were."""
+    vis, gcf, N = case(S, n=20000, seed=3)
+    origin, width, pitch = (16, 24), 224, 230
+    uvg = np.zeros((width, pitch, 2))
+    uvg[..., 1] = -0.25
+    exp = uvg.copy()
+    H.scatter(vis, gcf, exp, 0.9, N, 6, width=width, origin=origin)
+    rc = HB.scatter(0.9, N, 6, HB.vis_buffer(vis), HB.gcf_buffer(gcf),
+                    HB.uvg_buffer(uvg, width, origin), gpu1=gpu1)
+    assert rc == 0
+    assert close(uvg, exp)
+    assert np.array_equal(uvg[:, width:], exp[:, width:])
+    assert np.count_nonzero(exp[..., 0]) > S * S
+
+
 @pytest.mark.parametrize("S", [8, 32])
 @pytest.mark.parametrize("gpu1", [False, True])
 def test_degrid_vs_oracle(hip_device, S, gpu1):
