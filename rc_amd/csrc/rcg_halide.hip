@@ -456,12 +456,59 @@ __global__ void k_nv_zero(double *__restrict__ p, long long n)
     p[i] = 0.0;
 }
 
+// A point's taps land at x = mx + a - 7, y = my + b - 7 for a, b in [0, 16):
+// all of them inside the img x img grid.
+__device__ __forceinline__ bool nv_inside(int mx, int my, int img)
+{
+  return mx >= 7 && mx + 8 < img && my >= 7 && my + 8 < img;
+}
+
+// Sub-pixel offset floor(8 (xs - floor xs)), at most 7: for xs a hair below
+// an integer the difference rounds to 1.0, which would index the next row
+// of layers (and, in the last row, past the table).
+__device__ __forceinline__ int nv_sub(double xs, double fx)
+{
+  return min((int)floor(H_OVER * (xs - fx)), H_OVER - 1);
+}
+
+// Records of the tiled FP64 gridder for nvGridder: per point whose window
+// lies inside the grid, its first cell ((my - 7) * img + mx - 7, rows = y)
+// and the first tap of its 16 x 16 layer (8 sub_y + sub_x) * 256, tap (a,
+// b) at + 16 b + a; cell -1 otherwise (k_nv_scatter grids those).  The
+// visibility is packed as a complex double.
+__global__ void __launch_bounds__(256)
+k_nv_records(const double *__restrict__ pos, int ps, const double *__restrict__ val, int vs,
+             long long n, double scale, int img, rcg_pg *__restrict__ pg,
+             double2 *__restrict__ v2)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double xs = pos[t * ps] * scale, ys = pos[t * ps + 1] * scale;
+  const double fx = floor(xs), fy = floor(ys);
+  rcg_pg rec;
+  rec.cell = -1;
+  rec.tap0 = 0;
+  rec.stride = NV_GCF;
+  rec.supp = NV_GCF;
+  rec.flags = 0;
+  if (fx >= -8.0 && fx <= img + 6.0 && fy >= -8.0 && fy <= img + 6.0) {
+    const int mx = (int)fx, my = (int)fy;
+    if (nv_inside(mx, my, img)) {
+      rec.cell = (my - 7) * img + (mx - 7);
+      rec.tap0 = (H_OVER * nv_sub(ys, fy) + nv_sub(xs, fx)) * NV_TAPS;
+    }
+  }
+  pg[t] = rec;
+  v2[t] = make_double2(val[t * vs], val[t * vs + 1]);
+}
+
 // pos: x, y of point t at pos[t * ps], pos[t * ps + 1]; val: re, im at
-// val[t * vs], val[t * vs + 1]
+// val[t * vs], val[t * vs + 1].  skip_inside: leave out the points whose
+// window lies inside the grid (gridded by the tiled route).
 __global__ void __launch_bounds__(256)
 k_nv_scatter(const double *__restrict__ pos, int ps, const double *__restrict__ val, int vs,
              long long n, const double2 *__restrict__ gcf, double *__restrict__ out,
-             double scale, int img)
+             double scale, int img, int skip_inside)
 {
 #pragma clang fp contract(off)
   const long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -472,7 +519,8 @@ k_nv_scatter(const double *__restrict__ pos, int ps, const double *__restrict__ 
   // no tap of a point outside these bounds lands in the grid (also NaN)
   if (!(fx >= -8.0 && fx <= img + 6.0 && fy >= -8.0 && fy <= img + 6.0)) return;
   const int mx = (int)fx, my = (int)fy;
-  const int sx = (int)floor(H_OVER * (xs - fx)), sy = (int)floor(H_OVER * (ys - fy));
+  if (skip_inside && nv_inside(mx, my, img)) return;
+  const int sx = nv_sub(xs, fx), sy = nv_sub(ys, fy);
   const double2 *g = gcf + (H_OVER * sy + sx) * NV_TAPS;
   const double vr = val[t * vs], vi = val[t * vs + 1];
   for (int k = lane; k < NV_TAPS; k += 64) {
@@ -547,6 +595,7 @@ int nv_grid_host(const char *who, double scale, double *out, const double *pos, 
   const size_t gbytes = sizeof(double2) * H_OVER * H_OVER * NV_TAPS;
   const size_t obytes = sizeof(double) * 2 * (size_t)cells;
   int rc = 0;
+  PlanHold plan;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&dp), pbytes) != hipSuccess ||
       (!shared && hipMalloc(reinterpret_cast<void **>(&dv), vbytes) != hipSuccess) ||
@@ -566,11 +615,49 @@ int nv_grid_host(const char *who, double scale, double *out, const double *pos, 
       hipLaunchKernelGGL(k_nv_zero, dim3(zb), dim3(256), 0, st, dout, nd);
       e = hipGetLastError();
     }
+    // Points whose window lies inside the grid: binned by tile, FP64 tile
+    // sums in LDS added to the zeroed grid.  The rest (edges), or all points
+    // where no plan fits: one FP64 atomic per tap.
+    const double *vals = shared ? dp + 3 : dv;
+    int tiled = 0;
+    if (e == hipSuccess && n > 0 && n <= INT_MAX) {
+      rcg_geom g{};
+      // read only by rcg_tiled_bin's pregrid, which k_nv_records replaces
+      g.uvw = reinterpret_cast<const rcg_double3 *>(dp);
+      g.chunk_supps = reinterpret_cast<const int32_t *>(dp);
+      g.n_chunks = (int)n;
+      g.ts_ch = 1;
+      g.scale = scale;
+      g.wstep = 1.0;
+      g.grid_size = img;
+      g.grid_pitch = img;
+      for (int tile : {32, 64}) {
+        const long long nt = (img + tile - 1) / tile;
+        if (nt * nt > 16384) continue;
+        if (rcg_tiled_plan_create(&g, tile, 1024, NV_GCF, &plan.p) == RCG_OK) break;
+        plan.p = nullptr;
+      }
+      clear_error();
+      if (plan.p &&
+          hipMalloc(reinterpret_cast<void **>(&plan.v2), (size_t)n * sizeof(double2)) ==
+              hipSuccess) {
+        hipLaunchKernelGGL(k_nv_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           dp, ps, vals, vs, n, scale, img, plan.p->pg, plan.v2);
+        e = hipGetLastError();
+        if (e == hipSuccess &&
+            (tiled_bin_records(plan.p, st) != RCG_OK ||
+             tiled_grid_f64_add(plan.p, dg, (long long)H_OVER * H_OVER * NV_TAPS, plan.v2,
+                                reinterpret_cast<double2 *>(dout), st) != RCG_OK))
+          e = hipErrorLaunchFailure;
+        tiled = 1;
+      } else {
+        plan.v2 = nullptr;
+      }
+    }
     if (e == hipSuccess && n > 0) {
       const long long blocks = (n * 64 + 255) / 256;
-      const double *vals = shared ? dp + 3 : dv;
       hipLaunchKernelGGL(k_nv_scatter, dim3((unsigned)blocks), dim3(256), 0, st, dp, ps, vals,
-                         vs, n, dg, dout, scale, img);
+                         vs, n, dg, dout, scale, img, tiled);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, obytes, hipMemcpyDeviceToHost, st);
