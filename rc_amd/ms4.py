@@ -34,13 +34,21 @@ def _legacy(name: str, vc: VisChunks, gcf: GCFTable, grid: np.ndarray,
 
 
 def grid(vc: VisChunks, gcf: GCFTable, grid_size: int,
-         grid_pitch: int | None = None) -> tuple[np.ndarray, int]:
+         grid_pitch: int | None = None,
+         out: np.ndarray | None = None) -> tuple[np.ndarray, int]:
     """gridKernelCPU{Full,Half}GCF: returns (grid[size, pitch] complex128,
-    in-bound tap count)."""
+    in-bound tap count).  out: a C-contiguous complex128 [size, pitch]
+    array to grid into (overwritten), as a caller reuses its grid."""
     pitch = grid_pitch or grid_size
     if gcf.data.dtype != np.complex128:
         raise TypeError("the MS4 entry points take complex128 GCF taps")
-    g = np.empty((grid_size, pitch), np.complex128)
+    if out is None:
+        g = np.empty((grid_size, pitch), np.complex128)
+    elif out.dtype != np.complex128 or out.shape != (grid_size, pitch) or \
+            not out.flags.c_contiguous:
+        raise ValueError(f"out must be a C-contiguous complex128 ({grid_size}, {pitch}) array")
+    else:
+        g = out
     vis = np.ascontiguousarray(vc.vis, np.complex128)
     name = "gridKernelCPUHalfGCF" if gcf.half else "gridKernelCPUFullGCF"
     taps = _legacy(name, vc, gcf, g, vis, grid_size, pitch)
@@ -48,15 +56,21 @@ def grid(vc: VisChunks, gcf: GCFTable, grid_size: int,
 
 
 def degrid(vc: VisChunks, gcf: GCFTable, grid_arr: np.ndarray,
-           grid_size: int | None = None) -> tuple[np.ndarray, int]:
+           grid_size: int | None = None,
+           out: np.ndarray | None = None) -> tuple[np.ndarray, int]:
     """deGridKernelCPU{Full,Half}GCF: returns (vis[n_chunks, ts_ch]
-    complex128, in-bound tap count)."""
+    complex128, in-bound tap count).  out: a C-contiguous complex128
+    [n_chunks, ts_ch] array to write (overwritten)."""
     g = np.ascontiguousarray(grid_arr, np.complex128)
     size = grid_size or g.shape[0]
     pitch = g.shape[1]
     if gcf.data.dtype != np.complex128:
         raise TypeError("the MS4 entry points take complex128 GCF taps")
-    out = np.zeros((vc.n_chunks, vc.ts_ch), np.complex128)
+    if out is None:
+        out = np.zeros((vc.n_chunks, vc.ts_ch), np.complex128)
+    elif out.dtype != np.complex128 or out.shape != (vc.n_chunks, vc.ts_ch) or \
+            not out.flags.c_contiguous:
+        raise ValueError(f"out must be a C-contiguous complex128 ({vc.n_chunks}, {vc.ts_ch}) array")
     name = "deGridKernelCPUHalfGCF" if gcf.half else "deGridKernelCPUFullGCF"
     taps = _legacy(name, vc, gcf, g, out, size, pitch)
     return out, taps

@@ -142,5 +142,6 @@ class VisChunks:
             raise ValueError("array must be C-contiguous")
         stride = arr.strides[0]
         base = arr.ctypes.data
-        return (C.c_void_p * self.n_chunks)(
-            *[base + b * stride for b in range(self.n_chunks)])
+        # built in numpy; the ctypes array shares (and keeps alive) its memory
+        ptrs = np.uint64(base) + np.arange(self.n_chunks, dtype=np.uint64) * np.uint64(stride)
+        return (C.c_void_p * self.n_chunks).from_buffer(ptrs)
