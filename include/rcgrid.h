@@ -385,6 +385,14 @@ rcg_status rcg_fft_plan_create(int sign, int f64, int size, int pitch,
 rcg_status rcg_fft_execute(rcg_fft_plan *plan, void *data, void *stream);
 rcg_status rcg_fft_plan_destroy(rcg_fft_plan *plan);
 
+/* Releases process-wide library state before the process exits: the rocFFT
+ * library state that the first FFT plan set up (rocfft_setup) is cleaned up
+ * (rocfft_cleanup).  Refused with RCG_ERR_ARG while an FFT plan (of
+ * rcg_fft_plan_create, fft_inplace_even or a GCF generator) is alive.  A
+ * later FFT plan sets the state up again.  No counterpart in the reference,
+ * which never releases FFTW's state. */
+rcg_status rcg_shutdown(void);
+
 /* GCF generation on device memory (the reference's mkGCFLayer and
  * calcAccums, GCF.cpp:16-219).  A generator owns the (8 max_support)^2
  * FP64 arena with row pitch 8 max_support + src_pad and its rocFFT plan;

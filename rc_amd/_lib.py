@@ -6,8 +6,11 @@ call raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import sys
 import threading
+import weakref
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "librcgrid.so"
@@ -101,6 +104,7 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_fft_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_fft_execute": (C.c_int, [vp, vp, vp]),
     "rcg_fft_plan_destroy": (C.c_int, [vp]),
+    "rcg_shutdown": (C.c_int, []),
     "rcg_hogbom_scratch_bytes": (C.c_size_t, []),
     "rcg_hogbom": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_double,
                              C.c_double, vp, vp]),
@@ -169,7 +173,62 @@ def lib() -> C.CDLL:
                 fn.restype = res
                 fn.argtypes = args
             _lib = h
+            atexit.register(shutdown)
     return _lib
+
+
+class Handle:
+    """Base of the Python objects that own a librcgrid handle (plans, graphs,
+    FFT plans, GCF generators).  Subclasses name their destroy function in
+    DESTROY and keep the handle in self._h.  Every live owner is registered
+    so that shutdown() (an atexit hook) releases its device memory while the
+    HIP runtime is still up; after that, and while the interpreter is
+    finalising, close() and __del__ make no library call."""
+
+    DESTROY = ""
+    _h = None
+
+    def _own(self, h: C.c_void_p) -> None:
+        self._h = h
+        _owners.add(self)
+
+    def _release(self) -> None:
+        """Extra clean-up of a subclass after its handle is destroyed."""
+
+    def close(self) -> None:
+        h, self._h = self._h, C.c_void_p()
+        if h and not _shut:
+            getattr(lib(), self.DESTROY)(h)
+        self._release()
+
+    def __del__(self):
+        if _shut or sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_owners: "weakref.WeakSet[Handle]" = weakref.WeakSet()
+_shut = False
+
+
+def shutdown() -> None:
+    """Destroy every live handle, then release the library's process-wide
+    state (rcg_shutdown: rocFFT's rocfft_cleanup).  Registered with atexit
+    when the library is loaded; later calls do nothing."""
+    global _shut
+    if _shut:
+        return
+    for o in list(_owners):
+        try:
+            o.close()
+        except Exception:
+            pass
+    _shut = True
+    if _lib is not None:
+        _lib.rcg_shutdown()
 
 
 def last_error() -> tuple[int, str]:

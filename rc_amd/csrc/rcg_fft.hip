@@ -32,13 +32,38 @@ struct rcg_fft_plan {
   void *work = nullptr;
   size_t work_bytes = 0;
   void *scratch = nullptr;  // real transforms only
+  bool counted = false;     // holds a reference on the rocFFT library state
 };
 
 namespace rcg {
 namespace {
 
-std::once_flag g_rocfft_once;
-rocfft_status g_rocfft_setup = rocfft_status_success;
+// rocFFT library state: set up by the first plan, released by rcg_shutdown
+// once no plan is alive (rocfft_setup / rocfft_cleanup pair).
+std::mutex g_fft_mu;
+bool g_fft_ready = false;
+int g_fft_plans = 0;
+
+rcg_status fft_acquire()
+{
+  std::lock_guard<std::mutex> lk(g_fft_mu);
+  if (!g_fft_ready) {
+    const rocfft_status rs = rocfft_setup();
+    if (rs != rocfft_status_success) {
+      set_error(RCG_ERR_FFT, "rocfft_setup failed (%d)", (int)rs);
+      return RCG_ERR_FFT;
+    }
+    g_fft_ready = true;
+  }
+  g_fft_plans++;
+  return RCG_OK;
+}
+
+void fft_release()
+{
+  std::lock_guard<std::mutex> lk(g_fft_mu);
+  g_fft_plans--;
+}
 
 // real <-> scratch copies for the reference's in-place strides
 template <class R>
@@ -76,6 +101,7 @@ extern "C" rcg_status rcg_fft_plan_destroy(rcg_fft_plan *p)
   if (p->plan) rocfft_plan_destroy(p->plan);
   if (p->work) (void)hipFree(p->work);
   if (p->scratch) (void)hipFree(p->scratch);
+  if (p->counted) fft_release();
   delete p;
   return RCG_OK;
 }
@@ -97,16 +123,16 @@ extern "C" rcg_status rcg_fft_plan_create(int sign, int f64, int size, int pitch
     set_error(RCG_ERR_ARG, "fft needs an even size <= pitch (size %d, pitch %d)", size, pitch);
     return RCG_ERR_ARG;
   }
-  std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
-  if (g_rocfft_setup != rocfft_status_success) {
-    set_error(RCG_ERR_FFT, "rocfft_setup failed (%d)", (int)g_rocfft_setup);
-    return RCG_ERR_FFT;
-  }
   rcg_fft_plan *p = new (std::nothrow) rcg_fft_plan();
   if (!p) {
     set_error(RCG_ERR_NOMEM, "host allocation failed");
     return RCG_ERR_NOMEM;
   }
+  if (fft_acquire() != RCG_OK) {
+    delete p;
+    return RCG_ERR_FFT;
+  }
+  p->counted = true;
   p->sign = sign;
   p->f64 = f64 ? 1 : 0;
   p->size = size;
@@ -241,6 +267,21 @@ extern "C" rcg_status rcg_fft_execute(rcg_fft_plan *p, void *data, void *stream)
              : copy_rows<float>(data, pitch, rscr, n, n, n, st);
   if (s != RCG_OK) return s;
   return launch_fftshift(data, re, n, pitch, st);
+}
+
+extern "C" rcg_status rcg_shutdown(void)
+{
+  clear_error();
+  std::lock_guard<std::mutex> lk(g_fft_mu);
+  if (g_fft_plans > 0) {
+    set_error(RCG_ERR_ARG, "rcg_shutdown: %d FFT plans still alive", g_fft_plans);
+    return RCG_ERR_ARG;
+  }
+  if (g_fft_ready) {
+    (void)rocfft_cleanup();
+    g_fft_ready = false;
+  }
+  return RCG_OK;
 }
 
 // ------------------------------------------------ reference host entry points --

@@ -139,7 +139,7 @@ def degrid_plain_f64(geom: DeviceGeom, gcf: DeviceGCF, pg: torch.Tensor,
 STAGE_BIN, STAGE_GRID, STAGE_DEGRID = 1, 2, 4
 
 
-class TiledPlan:
+class TiledPlan(_lib.Handle):
     """A tiled FP32 gridding plan (rcg_tiled_plan): pregrid + uv-tile bins for
     one geometry or a batch of instances (a list of geometries with one grid
     shape), reused by grid and degrid.  For a batch, visibilities are the
@@ -164,23 +164,14 @@ class TiledPlan:
         _lib.check(_lib.lib().rcg_tiled_plan_create_batch(arr, len(geoms), tile, chunk,
                                                           self.max_support, C.byref(h)),
                    "rcg_tiled_plan_create")
-        self._h = h
+        self._own(h)
 
     @property
     def grid_shape(self) -> tuple:
         g = (self.geom.grid_size, self.geom.grid_pitch)
         return (self.n_inst,) + g if self.batch else g
 
-    def close(self) -> None:
-        if self._h:
-            _lib.lib().rcg_tiled_plan_destroy(self._h)
-            self._h = C.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+    DESTROY = "rcg_tiled_plan_destroy"
 
     def bin(self, gcf: DeviceGCF, stream=None) -> None:
         t = gcf.struct()
@@ -299,30 +290,23 @@ class TiledPlan:
         return Graph(h, keep=(self, gcf, vis, grid, vis_out))
 
 
-class Graph:
+class Graph(_lib.Handle):
     """An executable HIP graph (rcg_graph); keeps its plan and buffers alive."""
 
+    DESTROY = "rcg_graph_destroy"
+
     def __init__(self, h: C.c_void_p, keep=()):
-        self._h = h
         self._keep = keep
+        self._own(h)
 
     def launch(self, stream=None) -> None:
         _lib.check(_lib.lib().rcg_graph_launch(self._h, _stream(stream)), "rcg_graph_launch")
 
-    def close(self) -> None:
-        if self._h:
-            _lib.lib().rcg_graph_destroy(self._h)
-            self._h = C.c_void_p()
+    def _release(self) -> None:
         self._keep = ()
 
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
-
-class FFTPlan:
+class FFTPlan(_lib.Handle):
     """rcg_fft plan: the reference's fft_inplace_even transform (sign -1/+1
     C2C, 0 R2C, 2 C2R, fftshift before and after, unnormalised) on a device
     buffer of `size` rows of `pitch` complex elements."""
@@ -332,7 +316,7 @@ class FFTPlan:
         self.sign, self.size, self.pitch, self.f64 = sign, size, pitch or size, f64
         _lib.check(_lib.lib().rcg_fft_plan_create(sign, int(f64), size, self.pitch, C.byref(h)),
                    "rcg_fft_plan_create")
-        self._h = h
+        self._own(h)
 
     def __call__(self, data: torch.Tensor, stream=None) -> torch.Tensor:
         want = torch.complex128 if self.f64 else torch.complex64
@@ -343,16 +327,7 @@ class FFTPlan:
                    "rcg_fft_execute")
         return data
 
-    def close(self) -> None:
-        if self._h:
-            _lib.lib().rcg_fft_plan_destroy(self._h)
-            self._h = C.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+    DESTROY = "rcg_fft_plan_destroy"
 
 
 def hogbom(res: torch.Tensor, psf: torch.Tensor, niters: int, gain: float, threshold: float,

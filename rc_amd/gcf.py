@@ -19,14 +19,15 @@ from .device import DeviceGCF, _stream
 from .tables import OVER
 
 
-class GCFGenerator:
+class GCFGenerator(_lib.Handle):
     """rcg_gcf_gen: FP64 arena and rocFFT plan for one max_support/src_pad."""
 
     def __init__(self, max_support: int, src_pad: int = 4):
         h = C.c_void_p()
         _lib.check(_lib.lib().rcg_gcf_gen_create(max_support, src_pad, C.byref(h)),
                    "rcg_gcf_gen_create")
-        self._h, self.max_support, self.src_pad = h, max_support, src_pad
+        self.max_support, self.src_pad = max_support, src_pad
+        self._own(h)
 
     def layer(self, dst: torch.Tensor, support: int, t2: float, w: float,
               pad_quirk: bool = False, stream=None) -> torch.Tensor:
@@ -42,16 +43,7 @@ class GCFGenerator:
                    "rcg_gcf_gen_layer")
         return dst
 
-    def close(self) -> None:
-        if self._h:
-            _lib.lib().rcg_gcf_gen_destroy(self._h)
-            self._h = C.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+    DESTROY = "rcg_gcf_gen_destroy"
 
 
 def calc_accums(uvw: torch.Tensor, wstep: float, n_planes: int,
