@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tile", type=int, default=32)
     ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--layout", choices=["pairs", "halo", "halo16"], default="pairs",
+                    help="record layout of the tiled plan (see rcg_tiled_plan_create_ex)")
     ap.add_argument("--n-times", type=int, default=306)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -927,7 +929,8 @@ def main() -> int:
     geom = D.DeviceGeom.from_chunks(vc, N)
     dg32 = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
     vis32 = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).to(dev)
-    plan = D.TiledPlan(geom, tile=args.tile, chunk=args.chunk, max_support=64)
+    plan = D.TiledPlan(geom, tile=args.tile, chunk=args.chunk, max_support=64,
+                       layout=args.layout)
     grid32 = torch.empty((N, N), dtype=torch.complex64, device=dev)
     out32 = torch.empty(vc.n, dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream()

@@ -280,6 +280,24 @@ rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile, int chunk,
 rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_inst,
                                        int tile, int chunk, int max_support,
                                        rcg_tiled_plan **plan);
+/* The same with a choice of record layout.  RCG_LAYOUT_PAIRS (the layout of
+ * the two functions above): one record per (point, tile) that the point's
+ * window overlaps, tile 16, 32 or 64; the only layout rcg_grid_tiled_f64
+ * takes.  RCG_LAYOUT_HALO: one record per point whose support is at most 32
+ * (16 when max_support <= 16), binned in the tile of its window's first
+ * cell; larger windows are cut into ceil(S/32)^2 near-equal sub-windows,
+ * each binned likewise; tile 16 or 32.  The kernels then hold a tile plus a
+ * 31-cell (15-cell) halo in LDS, so every record is a whole window with its
+ * lane geometry fixed at compile time.  chunk = maximum records per work
+ * item.  Grids of more tiles per instance than the binning's LDS histogram
+ * holds (16384) are binned with global atomics; a plan holds at most 2^20
+ * tiles over all instances. */
+#define RCG_LAYOUT_PAIRS 0
+#define RCG_LAYOUT_HALO 1
+#define RCG_LAYOUT_HALO16 2 /* halo with sub-windows of at most 16 cells */
+rcg_status rcg_tiled_plan_create_ex(const rcg_geom *geoms, int n_inst,
+                                    int tile, int chunk, int max_support,
+                                    int layout, rcg_tiled_plan **plan);
 rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *plan);
 rcg_status rcg_tiled_bin(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                          void *stream);

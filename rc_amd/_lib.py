@@ -73,6 +73,8 @@ SIGNATURES: dict[str, tuple] = {
                                         C.POINTER(vp)]),
     "rcg_tiled_plan_create_batch": (C.c_int, [C.POINTER(Geom), C.c_int, C.c_int, C.c_int,
                                               C.c_int, C.POINTER(vp)]),
+    "rcg_tiled_plan_create_ex": (C.c_int, [C.POINTER(Geom), C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.POINTER(vp)]),
     "rcg_tiled_plan_destroy": (C.c_int, [vp]),
     "rcg_tiled_bin": (C.c_int, [vp, C.POINTER(Gcf), vp]),
     "rcg_tiled_timing": (C.c_int, [vp, C.c_int]),

@@ -16,10 +16,23 @@ constexpr int MAX_PLAN_TILES = 1 << 20;
 // Error word bit: a pair did not fit the plan's pair buffer (a chunk
 // support above the plan's max_support).
 constexpr int ERR_CAPACITY = 2;
-// Work-queue counters: [0, 8) gridding, [8, 16) degridding, one per XCD.
-constexpr int N_QUEUE = 16;
+// Work-queue counters: [0, 8) gridding, [8, 16) degridding, one per XCD;
+// [16] the halo item sort.
+constexpr int N_QUEUE = 17;
+
+// Records per work item of a halo-layout plan (its kernels sort an item's
+// records in LDS).
+constexpr int MAX_HALO_ITEM = 1024;
+
+// Record layouts of a plan (rcg_tiled_plan_create_ex).
+constexpr int LAYOUT_PAIRS = 0;  // one record per (point, tile) the window overlaps
+constexpr int LAYOUT_HALO = 1;   // one record per sub-window, in its origin's tile
 
 struct TiledDev {
+  int layout;        // LAYOUT_PAIRS or LAYOUT_HALO
+  int wmax, lw;      // halo: largest sub-window side (16 or 32) and its log2
+  int gbins;         // binning counts and ranks with global atomics (tiles
+                     // per instance above the LDS histogram)
   int T, lt, ntx;
   int tpi;           // tiles per instance (ntx * ntx)
   int n_inst;
@@ -44,6 +57,11 @@ struct TiledDev {
   float2 *partial;   // degridding: partial sums, [tile slot][point]
   int adj;           // gridding with conjugated taps (set per launch)
   float2 *lo;        // compensated flush: rounding errors of the atomic adds
+  float2 *ibuf;      // halo gridding: each work item's region, items below
+  int ibuf_cap;      // ibuf_cap (later items flush with atomics)
+  int item_cap;      // halo: bound on the number of work items
+  uint16_t *runs;    // halo: per work item, its records per run key (64)
+  float2 *vsort;     // halo gridding: the visibility of every record, in record order
 };
 
 // Event pairs kept per direction for rcg_tiled_kernel_times.
@@ -59,6 +77,23 @@ namespace rcg {
 // (rcg_tiled_plan::pg, one per point; cell -1 for a point left out) in place
 // of the pregrid that rcg_tiled_bin runs.
 rcg_status tiled_bin_records(rcg_tiled_plan *pl, hipStream_t st);
+
+// Halo-layout kernels (rcg_halo.hip): persistent workgroups per CU x CUs
+// for the grid and degrid kernels of tile T and sub-windows of at most
+// wmax cells, and their launches.
+rcg_status halo_blocks(int T, int wmax, int *grid_blocks, int *degrid_blocks);
+rcg_status halo_grid(const TiledDev &d, int blocks, const float2 *vis, const float2 *taps,
+                     long long n_taps, float2 *grid, hipStream_t st);
+// Cells of a tile's region (T + wmax - 1)^2: the item-buffer stride.
+inline long long halo_region_cells(int T, int wmax)
+{
+  return (long long)(T + wmax - 1) * (T + wmax - 1);
+}
+rcg_status halo_degrid(const TiledDev &d, int blocks, const float2 *grid, const float2 *taps,
+                       long long n_taps, float2 *vis, hipStream_t st);
+// Sorts every work item's records by run key in place and writes the
+// items' run tables (the last pass of binning a halo-layout plan).
+rcg_status halo_sort_items(const TiledDev &d, hipStream_t st);
 
 // FP64 gridding over the plan's pairs, added to the grid's contents.
 rcg_status tiled_grid_f64_add(rcg_tiled_plan *pl, const double2 *taps, long long n_taps,

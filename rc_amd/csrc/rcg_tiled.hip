@@ -33,6 +33,7 @@
 
 #include "rcg_internal.h"
 #include "rcg_tiled.h"
+#include "rcg_tiledev.h"
 
 namespace rcg {
 
@@ -143,29 +144,99 @@ __device__ __forceinline__ BinBlock bin_block(const TiledDev &d)
   return r;
 }
 
+// Sub-window k of n along a side of S cells: first cell and length.  The
+// lengths are near-equal and even, except that an odd S gives its last
+// sub-window the odd cell (the kernels read two columns per lane from
+// even widths).  With n = ceil(S / wmax) every length is at most wmax.
+__device__ __forceinline__ void sub_range(int S, int n, int k, int *off, int *len)
+{
+  const int P = S >> 1;  // cell pairs
+  const int base = P / n, extra = P - base * n;
+  *off = 2 * (k * base + min(k, extra));
+  *len = 2 * (base + (k < extra ? 1 : 0)) + ((S & 1) && k == n - 1 ? 1 : 0);
+}
+
+// Halo record: one sub-window of a point, binned in the tile that holds its
+// first cell, 16 bytes.
+//   x: r0 | c0 << 6 | (w-1) << 12 | (h-1) << 18 | slot << 24 | conj << 30
+//      | whole << 31
+//      r0, c0: the sub-window's first cell, relative to its tile (< T)
+//      w, h:   width and height (<= wmax <= 32)
+//      slot:   index of the sub-window among the point's (row-major)
+//      whole:  the sub-window is the whole window
+//   y: tap offset (elements) of the sub-window's first cell
+//   z: layer row stride (elements, < 65536)
+//   w: point index
+// The windows of S <= wmax are whole; larger ones are cut into
+// ceil(S / wmax)^2 near-equal sub-windows.
+//
+// point_records(r, i, d, f) calls f(tile, slot, record) for every record of
+// point i in the plan's layout (tile: index within the instance) and returns
+// their number, 0 for a point left out, or -1 when there are more than the
+// plan's d.mpp.
+template <class F>
+__device__ __forceinline__ int point_records(const rcg_pg &r, int i, const TiledDev &d, F f)
+{
+  if (r.cell < 0 || r.supp <= 0) return 0;
+  int u0, v0, tu0, tu1, tv0, tv1;
+  point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+  if (d.layout == LAYOUT_PAIRS) {
+    const int np = (tu1 - tu0 + 1) * (tv1 - tv0 + 1);
+    if (np > d.mpp) return -1;
+    int j = 0;
+    for (int a = tu0; a <= tu1; a++)
+      for (int b = tv0; b <= tv1; b++, j++)
+        f(a * d.ntx + b, j, make_pair(u0, v0, r, i, a, b, d.T, j));
+    return np;
+  }
+  const int S = r.supp, n = (S + d.wmax - 1) >> d.lw;
+  if (n * n > d.mpp) return -1;
+  const int conj = (r.flags & PG_CONJ) ? (1 << 30) : 0;
+  const int whole = n == 1 ? (int)0x80000000 : 0;
+  for (int a = 0; a < n; a++) {
+    int ro, h;
+    sub_range(S, n, a, &ro, &h);
+    const int uu = u0 + ro;
+    for (int b = 0; b < n; b++) {
+      int co, w;
+      sub_range(S, n, b, &co, &w);
+      const int vv = v0 + co;
+      int4 q;
+      q.x = (uu & (d.T - 1)) | ((vv & (d.T - 1)) << 6) | ((w - 1) << 12) | ((h - 1) << 18) |
+            ((a * n + b) << 24) | conj | whole;
+      q.y = r.tap0 + ro * r.stride + co;
+      q.z = r.stride;
+      q.w = i;
+      f((uu >> d.lt) * d.ntx + (vv >> d.lt), a * n + b, q);
+    }
+  }
+  return n * n;
+}
+
+// Counts per tile.  With LDS histograms (tiles per instance <= MAX_BIN_TILES)
+// each workgroup adds its histogram with one global atomic per touched tile;
+// otherwise every record adds one global atomic.
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin_count(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 {
   extern __shared__ int hist[];
-  for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) hist[t] = 0;
-  __syncthreads();
   const BinBlock bb = bin_block(d);
+  int *tc = d.tile_count + (long long)bb.inst * d.tpi;
+  if (!d.gbins) {
+    for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) hist[t] = 0;
+    __syncthreads();
+  }
   for (int k = 0; k < d.ppt; k++) {
     const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
     if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
-    if (r.cell < 0 || r.supp <= 0) continue;
-    int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) {
-      atomicOr(err, ERR_CAPACITY);  // support above the plan's max_support
-      continue;
-    }
-    for (int a = tu0; a <= tu1; a++)
-      for (int b = tv0; b <= tv1; b++) atomicAdd(&hist[a * d.ntx + b], 1);
+    const int nr = point_records(r, i, d, [&](int t, int, const int4 &) {
+      atomicAdd(d.gbins ? &tc[t] : &hist[t], 1);
+    });
+    if (nr < 0) atomicOr(err, ERR_CAPACITY);  // support above the plan's max_support
   }
+  if (d.gbins) return;
   __syncthreads();
-  int *tc = d.tile_count + (long long)bb.inst * d.tpi;
   for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS)
     if (hist[t]) atomicAdd(&tc[t], hist[t]);
 }
@@ -255,32 +326,43 @@ k_scan_apply(TiledDev d)
 }
 
 // Fill: LDS ranks within the workgroup, one global atomic per touched tile
-// for the workgroup's base, then every pair is written to its slot.
+// for the workgroup's base, then every record is written to its slot; with
+// global bins, one global atomic per record gives its slot.
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 {
   extern __shared__ int lds[];
+  const BinBlock bb = bin_block(d);
+  const long long t0 = (long long)bb.inst * d.tpi;
+  auto put = [&](long long pos, const int4 &q) {
+    if (pos < d.pair_cap) d.pairs[pos] = q;
+    else atomicOr(err, ERR_CAPACITY);
+  };
+  if (d.gbins) {
+    for (int k = 0; k < d.ppt; k++) {
+      const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
+      if (i >= bb.p1) break;
+      const rcg_pg r = pg[i];
+      point_records(r, i, d, [&](int t, int, const int4 &q) {
+        put((long long)d.tile_start[t0 + t] + atomicAdd(&d.tile_cursor[t0 + t], 1), q);
+      });
+    }
+    return;
+  }
   int *hist = lds;                                           // tpi
-  uint16_t *rank = reinterpret_cast<uint16_t *>(lds + d.tpi);  // per pair
+  uint16_t *rank = reinterpret_cast<uint16_t *>(lds + d.tpi);  // per record
   for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) hist[t] = 0;
   __syncthreads();
-  const BinBlock bb = bin_block(d);
   for (int k = 0; k < d.ppt; k++) {
     const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
     if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
-    if (r.cell < 0 || r.supp <= 0) continue;
-    int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) continue;
-    int j = 0;
-    for (int a = tu0; a <= tu1; a++)
-      for (int b = tv0; b <= tv1; b++, j++)
-        rank[(k * BIN_THREADS + threadIdx.x) * d.mpp + j] =
-            (uint16_t)atomicAdd(&hist[a * d.ntx + b], 1);
+    uint16_t *rk = rank + (k * BIN_THREADS + threadIdx.x) * d.mpp;
+    point_records(r, i, d, [&](int t, int j, const int4 &) {
+      rk[j] = (uint16_t)atomicAdd(&hist[t], 1);
+    });
   }
   __syncthreads();
-  const long long t0 = (long long)bb.inst * d.tpi;
   for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) {
     const int c = hist[t];
     hist[t] = c ? d.tile_start[t0 + t] + atomicAdd(&d.tile_cursor[t0 + t], c) : 0;
@@ -290,125 +372,16 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
     const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
     if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
-    if (r.cell < 0 || r.supp <= 0) continue;
-    int u0, v0, tu0, tu1, tv0, tv1;
-    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-    if ((tu1 - tu0 + 1) * (tv1 - tv0 + 1) > d.mpp) continue;
-    int j = 0;
-    for (int a = tu0; a <= tu1; a++)
-      for (int b = tv0; b <= tv1; b++, j++) {
-        const long long pos = (long long)hist[a * d.ntx + b] +
-                              rank[(k * BIN_THREADS + threadIdx.x) * d.mpp + j];
-        if (pos < d.pair_cap) d.pairs[pos] = make_pair(u0, v0, r, i, a, b, 1 << d.lt, j);
-        else atomicOr(err, ERR_CAPACITY);
-      }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// work items
-
-// The largest t with item_start[t] <= item.  Tiles after the owning tile
-// start past `item`, so this is the owner (empty tiles before it share
-// smaller indices).
-__device__ __forceinline__ int find_tile(const int *__restrict__ item_start, int n_tiles,
-                                         int item)
-{
-  int lo = 0, hi = n_tiles - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (item_start[mid] <= item) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Next work item of a workgroup (called by one thread).  The `total` items
-// are cut into 8 contiguous ranges, one per XCD: workgroups are dispatched
-// round-robin over the XCDs, so workgroup b runs on XCD b mod 8 and draws
-// from range b mod 8 first, then from the others in turn (`tried` counts
-// the ranges found empty).  Returns -1 when every range is used up; each
-// counter only grows, so every workgroup ends after at most 8 failed draws.
-__device__ __forceinline__ int draw_item(int *__restrict__ q, int total, int home, int &tried)
-{
-  const int per = (total + 7) >> 3;
-  while (tried < 8) {
-    const int x = (home + tried) & 7;
-    const int k = atomicAdd(&q[x], 1);
-    if (k < per && x * per + k < total) return x * per + k;
-    tried++;
-  }
-  return -1;
-}
-
-struct ItemRange {
-  int begin, end, single;  // pairs [begin, end); the tile has one item
-  int tu0, tv0;            // first grid row and column of the tile
-  long long goff;          // element offset of the instance's grid
-};
-
-__device__ __forceinline__ ItemRange item_of(const TiledDev &d, int item)
-{
-  const int t = find_tile(d.item_start, d.n_tiles, item);
-  const int k = item - d.item_start[t];
-  const int cnt = d.tile_count[t];
-  const int inst = t / d.tpi, lt = t - inst * d.tpi;
-  ItemRange ir;
-  ir.begin = d.tile_start[t] + k * d.chunk;
-  ir.end = d.tile_start[t] + min(cnt, (k + 1) * d.chunk);
-  ir.single = cnt <= d.chunk;
-  ir.tu0 = (lt / d.ntx) * d.T;
-  ir.tv0 = (lt % d.ntx) * d.T;
-  ir.goff = (long long)inst * d.gstride;
-  return ir;
-}
-
-// Work loop of a persistent workgroup: fn(item range) for every item it
-// draws.  The trailing barrier keeps the next draw (and the next item's
-// LDS writes) behind every wavefront's use of the current one.
-template <class F>
-__device__ __forceinline__ void for_each_item(const TiledDev &d, int *__restrict__ q, F fn)
-{
-  __shared__ int s_item;
-  const int total = d.totals[1];
-  int tried = 0;
-  while (true) {
-    if (threadIdx.x == 0) s_item = draw_item(q, total, blockIdx.x & 7, tried);
-    __syncthreads();
-    const int item = s_item;
-    if (item < 0) break;
-    fn(item_of(d, item));
-    __syncthreads();
+    const uint16_t *rk = rank + (k * BIN_THREADS + threadIdx.x) * d.mpp;
+    point_records(r, i, d, [&](int t, int j, const int4 &q) {
+      put((long long)hist[t] + rk[j], q);
+    });
   }
 }
 
 // ---------------------------------------------------------------------------
 // Walking a pair's window/tile intersection.
-
-constexpr unsigned TAP_OOB = 0x7ffffff0u;  // byte offset past any tap table
-
-__device__ __forceinline__ int rl(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
-__device__ __forceinline__ float rlf(float v, int j)
-{
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t tap_rsrc(const float2 *taps, long long n)
-{
-  // raw buffer over the tap table; reads past num_records return 0
-  const long long bytes = n * 8 < 0x7fffffffLL ? n * 8 : 0x7fffffffLL;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(taps), (short)0,
-                                           (int)bytes, 0x00020000);
-}
-
-// voff: per-lane byte offset (checked against the table size); soff:
-// uniform byte offset added to it.
-__device__ __forceinline__ float2 tap_load(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
-{
-  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  return make_float2(__int_as_float(v[0]), __int_as_float(v[1]));
-}
-
+//
 // Per pair: the lanes form `rows` rows of `w` columns (lane l at row
 // l / w, column l % w; lanes past rows*w idle); slab s covers intersection
 // rows [s*rows, (s+1)*rows), the last one possibly partial.  A lane's tap
@@ -492,41 +465,6 @@ __device__ __forceinline__ void load_group(float2 (&t)[GRP], int (&la)[GRP], con
     t[u] = tap_load(rsrc, act ? p.tb0 : TAP_OOB, (s0 + u) * p.tstep);
     la[u] = act ? p.lb0 + (s0 + u) * p.lstep : dummy;
   }
-}
-
-__device__ __forceinline__ float2 &lds_at(float2 *lds, int byte)
-{
-  return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + byte);
-}
-
-// A wavefront's pairs are ir.begin + wave + nw k (nw wavefronts in the
-// workgroup), each cut into groups of GRP
-// slabs.  Their records (and visibilities) come 64 at a time, one per lane,
-// double-buffered a batch ahead and broadcast with readlane; batch b is
-// loaded by stream_batch.
-template <bool VIS>
-__device__ __forceinline__ void stream_batch(const TiledDev &d, const float2 *__restrict__ vis,
-                                             const ItemRange &ir, int wave, int lane, int npw,
-                                             int b, int4 &rec, float2 &v, int nw)
-{
-  const int k = b * 64 + lane;
-  rec = make_int4(0, 0, 0, 0);
-  v = make_float2(0.f, 0.f);
-  if (k < npw) {
-    rec = d.pairs[ir.begin + wave + nw * k];
-    if (VIS) v = vis[rec.w];
-  }
-}
-
-// Error of the rounded sum fl(a + b) (Knuth's TwoSum): fl(a + b) + err =
-// a + b exactly, in round-to-nearest without overflow.
-__device__ __forceinline__ float two_sum_err(float a, float b)
-{
-#pragma clang fp contract(off)
-  const float s = a + b;
-  const float bp = s - a;
-  const float ap = s - bp;
-  return (a - ap) + (b - bp);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,67 +745,6 @@ __global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restri
 // lanes accumulate four partial products, which are summed across the
 // wavefront four pairs at a time.
 
-// Sums of 4 pairs at once.  Pair k of a wavefront (in the order it walks
-// them) has its output code in codes[k % 128], written by lane k % 64 when
-// its batch of 64 records becomes current, and when it ends every lane
-// leaves its part of the pair's sum in row k % 4 of the wavefront's LDS
-// buffer.  After
-// pairs 4a..4a+3 (or at the end of an item) lane l sums entries c, c + 16,
-// c + 32, c + 48 of row l / 16 (c = l % 16), then a DPP prefix sum over its
-// row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill) leaves the pair's
-// total in the row's last lane, which stores it.  The order of the
-// additions is fixed, so the result is reproducible.
-constexpr int SUMS = 4;
-// Output addresses are typed global, so the final store is a global (not
-// flat) store and does not hold up the wavefront's LDS waits; the (re, im)
-// pair is stored as one 64-bit word, re in the low half.
-typedef __attribute__((address_space(1))) unsigned long long gu64;
-
-__device__ __forceinline__ float row_sum(float x)
-{
-  int v = __float_as_int(x);
-  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
-  v = __float_as_int(x);
-  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
-  v = __float_as_int(x);
-  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
-  v = __float_as_int(x);
-  x += __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
-  return x;
-}
-
-// Output of a pair, from its record: ~point for a point whose window lies
-// in one tile (its visibility), slot * npts + point for a split point (its
-// partial); below 2^31 because plans hold at most 2^31 - 1 pairs.
-__device__ __forceinline__ int out_code(const int4 &rec, int npts)
-{
-  const int whole = rec.x >> 31;  // all ones for a whole pair
-  const int split = ((rec.x >> 24) & 63) * npts + rec.w;
-  return (~rec.w & whole) | (split & ~whole);
-}
-
-// n: pairs in the buffer (1..SUMS), whose output codes are code[0..n);
-// entries of the other rows are stale and their sums are not stored
-__device__ __forceinline__ void flush_sums(const float2 *buf, const int *code, float2 *vis,
-                                           float2 *partial, int n, int lane)
-{
-  // the entries come from other lanes of the wavefront: keep the compiler
-  // from moving LDS accesses across this point (the hardware runs a
-  // wavefront's LDS instructions in order)
-  __builtin_amdgcn_wave_barrier();
-  const int q = lane >> 4, c = lane & 15;
-  const float2 *b = buf + q * 64 + c;
-  const float2 a0 = b[0], a1 = b[16], a2 = b[32], a3 = b[48];
-  const float re = row_sum((a0.x + a1.x) + (a2.x + a3.x));
-  const float im = row_sum((a0.y + a1.y) + (a2.y + a3.y));
-  if (c == 15 && q < n) {
-    const int o = code[q];
-    gu64 *dst = (gu64 *)(o < 0 ? vis + ~o : partial + o);
-    *dst = (unsigned long long)__float_as_uint(im) << 32 | __float_as_uint(re);
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
 template <int T>
 __global__ void __launch_bounds__(256)
 k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
@@ -1008,9 +885,15 @@ k_degrid_reduce(TiledDev d, const rcg_pg *__restrict__ pg, long long n,
   if (i >= n) return;
   const rcg_pg r = pg[i];
   if (r.cell < 0 || r.supp <= 0) return;
-  int u0, v0, tu0, tu1, tv0, tv1;
-  point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
-  const int np = (tu1 - tu0 + 1) * (tv1 - tv0 + 1);
+  int np;
+  if (d.layout == LAYOUT_HALO) {
+    const int ns = (r.supp + d.wmax - 1) >> d.lw;
+    np = ns * ns;
+  } else {
+    int u0, v0, tu0, tu1, tv0, tv1;
+    point_tiles(r, d.pitch, d.inv_pitch, d.lt, &u0, &v0, &tu0, &tu1, &tv0, &tv1);
+    np = (tu1 - tu0 + 1) * (tv1 - tv0 + 1);
+  }
   if (np <= 1 || np > d.mpp) return;  // whole (written by the tile kernel)
   // partials are stored slot-major, so consecutive points read consecutive
   // elements of each slot
@@ -1089,9 +972,9 @@ hipEvent_t *timing_slot(rcg_tiled_plan *pl, int dir)
 
 using namespace rcg;
 
-extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_inst, int tile,
-                                                  int chunk, int max_support,
-                                                  rcg_tiled_plan **out)
+extern "C" rcg_status rcg_tiled_plan_create_ex(const rcg_geom *geoms, int n_inst, int tile,
+                                               int chunk, int max_support, int layout,
+                                               rcg_tiled_plan **out)
 {
   clear_error();
   if (!geoms || !out || n_inst < 1) {
@@ -1099,6 +982,11 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
     return RCG_ERR_ARG;
   }
   *out = nullptr;
+  if (layout != RCG_LAYOUT_PAIRS && layout != RCG_LAYOUT_HALO && layout != RCG_LAYOUT_HALO16) {
+    set_error(RCG_ERR_ARG, "layout %d is not one of RCG_LAYOUT_PAIRS, RCG_LAYOUT_HALO, "
+              "RCG_LAYOUT_HALO16", layout);
+    return RCG_ERR_ARG;
+  }
   long long n = 0;
   std::vector<int> pt_off(n_inst + 1), blk_off(n_inst + 1);
   for (int k = 0; k < n_inst; k++) {
@@ -1116,48 +1004,58 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
     }
   }
   pt_off[n_inst] = (int)n;
-  if (tile != 16 && tile != 32 && tile != 64) {
-    set_error(RCG_ERR_ARG, "tile must be 16, 32 or 64 (got %d)", tile);
+  const bool halo = layout != RCG_LAYOUT_PAIRS;
+  if (tile != 16 && tile != 32 && (halo || tile != 64)) {
+    set_error(RCG_ERR_ARG, "tile must be 16, 32%s (got %d)", halo ? "" : " or 64", tile);
     return RCG_ERR_ARG;
   }
-  if (chunk < 1 || chunk > (1 << 20) || max_support < 1 || max_support > MAX_SUPPORT) {
-    set_error(RCG_ERR_ARG, "chunk %d / max_support %d out of range", chunk, max_support);
+  if (chunk < 1 || chunk > (halo ? MAX_HALO_ITEM : (1 << 20)) || max_support < 1 ||
+      max_support > MAX_SUPPORT) {
+    set_error(RCG_ERR_ARG, "chunk %d / max_support %d out of range%s", chunk, max_support,
+              halo ? " (the halo layout takes chunk <= 1024)" : "");
     return RCG_ERR_ARG;
   }
   const int size = geoms[0].grid_size, pitch = geoms[0].grid_pitch;
   const int ntx = (size + tile - 1) / tile;
   const long long tpi = (long long)ntx * ntx;
-  if (tpi > MAX_BIN_TILES) {
-    set_error(RCG_ERR_ARG, "%lld tiles per grid exceed the binning limit %d: use a larger tile",
-              tpi, MAX_BIN_TILES);
-    return RCG_ERR_ARG;
-  }
   if (tpi * n_inst > MAX_PLAN_TILES) {
-    set_error(RCG_ERR_ARG, "%lld tiles in the plan exceed %d: split the batch",
-              tpi * n_inst, MAX_PLAN_TILES);
+    set_error(RCG_ERR_ARG, "%lld tiles in the plan exceed %d: use a larger tile or split "
+              "the batch", tpi * n_inst, MAX_PLAN_TILES);
     return RCG_ERR_ARG;
   }
-  const int per_dim = (max_support - 1 + tile - 1) / tile + 1;
-  const int max_pairs_pt = per_dim * per_dim;
-  if (max_pairs_pt > 64) {  // the pair record holds the tile slot in 6 bits
+  // records per point: tiles a window overlaps (pairs) or sub-windows (halo)
+  const int wmax = max_support <= 16 || layout == RCG_LAYOUT_HALO16 ? 16 : 32;
+  int max_pairs_pt;
+  if (halo) {
+    const int nsub = (max_support + wmax - 1) / wmax;
+    max_pairs_pt = nsub * nsub;
+  } else {
+    const int per_dim = (max_support - 1 + tile - 1) / tile + 1;
+    max_pairs_pt = per_dim * per_dim;
+  }
+  if (max_pairs_pt > 64) {  // the record holds the slot in 6 bits
     set_error(RCG_ERR_UNSUPPORTED,
-              "max_support %d spans %d tiles of %d; at most 64 are supported",
+              "max_support %d needs %d records per point with tile %d; at most 64 are supported",
               max_support, max_pairs_pt, tile);
     return RCG_ERR_UNSUPPORTED;
   }
   if (n * max_pairs_pt > INT_MAX) {
-    set_error(RCG_ERR_ARG, "%lld points x %d tiles per point exceed the pair index range: "
+    set_error(RCG_ERR_ARG, "%lld points x %d records per point exceed the record index range: "
               "split the batch", n, max_pairs_pt);
     return RCG_ERR_ARG;
   }
-  // points per binning thread: as many as the fill pass's LDS allows
+  // Binning: LDS histograms per workgroup when the instance's tiles and the
+  // fill pass's ranks fit in LDS (points per thread as many as fit), global
+  // atomics per record otherwise.
   int ppt = 4;
-  while (ppt > 0 && tpi * 4 + (long long)BIN_THREADS * ppt * max_pairs_pt * 2 > 160 * 1024)
-    ppt >>= 1;
-  if (ppt == 0) {
-    set_error(RCG_ERR_UNSUPPORTED, "binning LDS: %lld tiles and %d tiles per point do not fit",
-              tpi, max_pairs_pt);
-    return RCG_ERR_UNSUPPORTED;
+  bool gbins = tpi > MAX_BIN_TILES;
+  if (!gbins) {
+    while (ppt > 0 && tpi * 4 + (long long)BIN_THREADS * ppt * max_pairs_pt * 2 > 160 * 1024)
+      ppt >>= 1;
+    if (ppt == 0) {
+      gbins = true;
+      ppt = 4;
+    }
   }
   int nblk = 0;
   for (int k = 0; k < n_inst; k++) {
@@ -1177,6 +1075,10 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
   pl->max_pairs_pt = max_pairs_pt;
   pl->bin_blocks = nblk;
   TiledDev &d = pl->dev;
+  d.layout = halo ? LAYOUT_HALO : LAYOUT_PAIRS;
+  d.wmax = wmax;
+  d.lw = ilog2(wmax);
+  d.gbins = gbins ? 1 : 0;
   d.T = tile;
   d.lt = ilog2(tile);
   d.ntx = ntx;
@@ -1198,13 +1100,16 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
   if (e == hipSuccess)
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bin_count),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e == hipSuccess) {
+  if (e == hipSuccess && !halo) {
     switch (tile) {
       case 16: e = tile_kernels<16>(&pl->grid_blocks, &pl->degrid_blocks); break;
       case 32: e = tile_kernels<32>(&pl->grid_blocks, &pl->degrid_blocks); break;
       default: e = tile_kernels<64>(&pl->grid_blocks, &pl->degrid_blocks); break;
     }
   }
+  if (e == hipSuccess && halo &&
+      halo_blocks(tile, wmax, &pl->grid_blocks, &pl->degrid_blocks) != RCG_OK)
+    e = hipErrorInvalidValue;
   if (e == hipSuccess) e = dalloc(&pl->geoms_dev, (size_t)n_inst);
   if (e == hipSuccess) e = dalloc(&pl->offs, (size_t)2 * (n_inst + 1));
   if (e == hipSuccess) e = dalloc(&pl->pg, (size_t)n);
@@ -1219,6 +1124,19 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
   if (e == hipSuccess) e = dalloc(&pl->err, 1);
   if (e == hipSuccess) e = dalloc(&d.pairs, (size_t)d.pair_cap);
   if (e == hipSuccess) e = dalloc(&d.partial, (size_t)d.pair_cap);
+  if (e == hipSuccess && halo) {
+    // item buffers of the halo gridder: every work item up to the bound on
+    // their number (records / chunk + tiles), within a 2 GiB budget; items
+    // past it flush with atomics
+    const long long cells = halo_region_cells(tile, wmax);
+    const long long bound = d.pair_cap / chunk + d.n_tiles;
+    const long long budget = (2LL << 30) / (cells * (long long)sizeof(float2));
+    d.item_cap = (int)std::min(bound, (long long)INT_MAX);
+    d.ibuf_cap = (int)std::min(bound, budget);
+    e = dalloc(&d.ibuf, (size_t)(d.ibuf_cap * cells));
+    if (e == hipSuccess) e = dalloc(&d.runs, (size_t)d.item_cap * 64);
+    if (e == hipSuccess) e = dalloc(&d.vsort, (size_t)d.pair_cap);
+  }
   if (e == hipSuccess)
     e = hipMemcpy(pl->geoms_dev, geoms, sizeof(rcg_geom) * n_inst, hipMemcpyHostToDevice);
   if (e == hipSuccess)
@@ -1237,6 +1155,14 @@ extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_i
   return RCG_OK;
 }
 
+extern "C" rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_inst, int tile,
+                                                  int chunk, int max_support,
+                                                  rcg_tiled_plan **out)
+{
+  return rcg_tiled_plan_create_ex(geoms, n_inst, tile, chunk, max_support, RCG_LAYOUT_PAIRS,
+                                  out);
+}
+
 extern "C" rcg_status rcg_tiled_plan_create(const rcg_geom *geom, int tile, int chunk,
                                             int max_support, rcg_tiled_plan **out)
 {
@@ -1249,7 +1175,7 @@ extern "C" rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *pl)
   TiledDev &d = pl->dev;
   void *ptrs[] = {pl->geoms_dev, pl->offs, pl->pg, d.tile_count, d.tile_start,
                   d.tile_cursor, d.item_start, d.totals, d.seg, d.queue, pl->taps, pl->err,
-                  d.pairs, d.partial, pl->lo};
+                  d.pairs, d.partial, pl->lo, d.ibuf, d.runs, d.vsort};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int dir = 0; dir < 2; dir++)
@@ -1310,9 +1236,9 @@ namespace {
 rcg_status bin_passes(rcg_tiled_plan *pl, hipStream_t st)
 {
   TiledDev &d = pl->dev;
-  const size_t lds_count = (size_t)d.tpi * sizeof(int);
+  const size_t lds_count = d.gbins ? 0 : (size_t)d.tpi * sizeof(int);
   const size_t lds_fill =
-      lds_count + (size_t)BIN_THREADS * d.ppt * d.mpp * sizeof(uint16_t);
+      d.gbins ? 0 : lds_count + (size_t)BIN_THREADS * d.ppt * d.mpp * sizeof(uint16_t);
   const int n_seg = ceil_div(d.n_tiles, SCAN_SEG);
   hipLaunchKernelGGL(k_bin_count, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_count, st,
                      pl->pg, d, pl->err);
@@ -1322,6 +1248,11 @@ rcg_status bin_passes(rcg_tiled_plan *pl, hipStream_t st)
   hipLaunchKernelGGL(k_bin_fill, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_fill, st,
                      pl->pg, d, pl->err);
   RCG_HIP_TRY(hipGetLastError());
+  if (d.layout == LAYOUT_HALO) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue + 16, 0, 1LL);
+    RCG_HIP_TRY(hipGetLastError());
+    return halo_sort_items(d, st);
+  }
   return RCG_OK;
 }
 
@@ -1358,16 +1289,24 @@ rcg_status grid_tiled(rcg_tiled_plan *pl, const rcg_gcf *gcf, const rcg_c64 *vis
   const dim3 nb(pl->grid_blocks);
   hipEvent_t *ev = timing_slot(pl, 0);
   if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
-  switch (d.T) {
-    case 16:
-      hipLaunchKernelGGL(k_grid_tiles<16>, nb, dim3(64 * grid_waves<16>()), 0, st, d, v, t, nt, g);
-      break;
-    case 32:
-      hipLaunchKernelGGL(k_grid_tiles<32>, nb, dim3(64 * grid_waves<32>()), 0, st, d, v, t, nt, g);
-      break;
-    default:
-      hipLaunchKernelGGL(k_grid_tiles<64>, nb, dim3(64 * grid_waves<64>()), 0, st, d, v, t, nt, g);
-      break;
+  if (d.layout == LAYOUT_HALO) {
+    s = halo_grid(d, pl->grid_blocks, v, t, nt, g, st);
+    if (s != RCG_OK) return s;
+  } else {
+    switch (d.T) {
+      case 16:
+        hipLaunchKernelGGL(k_grid_tiles<16>, nb, dim3(64 * grid_waves<16>()), 0, st, d, v, t, nt,
+                           g);
+        break;
+      case 32:
+        hipLaunchKernelGGL(k_grid_tiles<32>, nb, dim3(64 * grid_waves<32>()), 0, st, d, v, t, nt,
+                           g);
+        break;
+      default:
+        hipLaunchKernelGGL(k_grid_tiles<64>, nb, dim3(64 * grid_waves<64>()), 0, st, d, v, t, nt,
+                           g);
+        break;
+    }
   }
   RCG_HIP_TRY(hipGetLastError());
   if (ev) {
@@ -1440,10 +1379,15 @@ extern "C" rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gc
   const dim3 nb(pl->degrid_blocks);
   hipEvent_t *ev = timing_slot(pl, 1);
   if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
-  switch (d.T) {
-    case 16: hipLaunchKernelGGL(k_degrid_tiles<16>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
-    case 32: hipLaunchKernelGGL(k_degrid_tiles<32>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
-    default: hipLaunchKernelGGL(k_degrid_tiles<64>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+  if (d.layout == LAYOUT_HALO) {
+    s = halo_degrid(d, pl->degrid_blocks, g, t, nt, v, st);
+    if (s != RCG_OK) return s;
+  } else {
+    switch (d.T) {
+      case 16: hipLaunchKernelGGL(k_degrid_tiles<16>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+      case 32: hipLaunchKernelGGL(k_degrid_tiles<32>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+      default: hipLaunchKernelGGL(k_degrid_tiles<64>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+    }
   }
   RCG_HIP_TRY(hipGetLastError());
   if (ev) {
@@ -1475,6 +1419,10 @@ rcg_status grid_f64(rcg_tiled_plan *pl, const double2 *v, const double2 *t, long
                     double2 *g, int add, hipStream_t st)
 {
   TiledDev d = pl->dev;
+  if (d.layout != LAYOUT_PAIRS) {
+    set_error(RCG_ERR_UNSUPPORTED, "FP64 tiled gridding needs a plan of RCG_LAYOUT_PAIRS");
+    return RCG_ERR_UNSUPPORTED;
+  }
   d.adj = 0;
   d.lo = nullptr;
   if (!add)

@@ -147,8 +147,10 @@ class TiledPlan(_lib.Handle):
     [n_inst, grid_size, grid_pitch].  Device scratch is owned by the library
     and released by close()."""
 
+    LAYOUTS = {"pairs": 0, "halo": 1, "halo16": 2}
+
     def __init__(self, geom, tile: int = 32, chunk: int = 512,
-                 max_support: int | None = None):
+                 max_support: int | None = None, layout: str = "pairs"):
         geoms = list(geom) if isinstance(geom, (list, tuple)) else [geom]
         self.batch = isinstance(geom, (list, tuple))
         self.geoms = geoms
@@ -159,10 +161,14 @@ class TiledPlan(_lib.Handle):
             max_support = max((int(g.chunk_supps.max().item()) for g in geoms if g.n_chunks),
                               default=1)
         self.tile, self.chunk, self.max_support = tile, chunk, max(1, max_support)
+        if layout not in self.LAYOUTS:
+            raise ValueError(f"layout must be one of {sorted(self.LAYOUTS)}")
+        self.layout = layout
         h = C.c_void_p()
         arr = (_lib.Geom * len(geoms))(*[g.struct() for g in geoms])
-        _lib.check(_lib.lib().rcg_tiled_plan_create_batch(arr, len(geoms), tile, chunk,
-                                                          self.max_support, C.byref(h)),
+        _lib.check(_lib.lib().rcg_tiled_plan_create_ex(arr, len(geoms), tile, chunk,
+                                                       self.max_support, self.LAYOUTS[layout],
+                                                       C.byref(h)),
                    "rcg_tiled_plan_create")
         self._own(h)
 

@@ -25,10 +25,10 @@ def c64_round(vc, gcf):
     return vc32, gcf.astype(np.complex64).astype(np.complex128)
 
 
-def run_tiled(vc, gcf, N, tile, chunk, grid_in=None):
+def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs"):
     geom = D.DeviceGeom.from_chunks(vc, N)
     dg = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
-    plan = D.TiledPlan(geom, tile=tile, chunk=chunk)
+    plan = D.TiledPlan(geom, tile=tile, chunk=chunk, layout=layout)
     plan.bin(dg)
     vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).cuda()
     g = plan.grid(dg, vis)
@@ -41,11 +41,18 @@ def run_tiled(vc, gcf, N, tile, chunk, grid_in=None):
     return out
 
 
-@pytest.mark.parametrize("tile", [16, 32, 64])
+LAYOUT_TILES = [("pairs", 16), ("pairs", 32), ("pairs", 64), ("halo", 16), ("halo", 32)]
+
+
+@pytest.mark.parametrize("layout,tile", LAYOUT_TILES)
 @pytest.mark.parametrize("chunk", [3, 512])
 @pytest.mark.parametrize("seed,half,cs", [(0, False, "min"), (1, True, "min"),
                                           (2, True, "random")])
-def test_tiled_random_vs_oracle(hip_device, tile, chunk, seed, half, cs):
+def test_tiled_random_vs_oracle(hip_device, layout, tile, chunk, seed, half, cs):
+    """Supports 6, 17 and 40 with truncated windows (chunk support below the
+    layer's) and, with 'random', odd chunk supports of every size up to the
+    plane's: square whole windows, and in the halo layout near-square
+    sub-windows of the 33..40-cell windows."""
     vc, gcf, _ = synth.random_small(seed, grid_size=160, chunks=30, ts_ch=20,
                                     half=half, chunk_supp=cs, supports=(6, 17, 40))
     N = 160
@@ -53,14 +60,15 @@ def test_tiled_random_vs_oracle(hip_device, tile, chunk, seed, half, cs):
     ref, taps = O.grid(vc, gcf, N, nthreads=4)
     ref32 = ref.astype(np.complex64).astype(np.complex128)
     vref, _ = O.degrid(vc, gcf, ref32, nthreads=4)
-    g, v, st = run_tiled(vc, gcf, N, tile, chunk, grid_in=ref32)
+    g, v, st = run_tiled(vc, gcf, N, tile, chunk, grid_in=ref32, layout=layout)
     assert st["err"] == 0
     assert st["taps"] == taps
     assert rel_l2(g, ref) <= TOL
     assert rel_l2(v, vref) <= TOL
 
 
-def test_tiled_wproj_slice_vs_oracle(hip_device):
+@pytest.mark.parametrize("layout,tile", [("pairs", 64), ("halo", 16), ("halo", 32)])
+def test_tiled_wproj_slice_vs_oracle(hip_device, layout, tile):
     """128k points of the earth-rotation workload on the full 4096^2 grid
     with 32 w-planes (supports 8..64)."""
     vc, gcf, N = synth.wproj(n_ant=64, n_times=64, vis_dtype=np.complex128)
@@ -68,7 +76,7 @@ def test_tiled_wproj_slice_vs_oracle(hip_device):
     ref, taps = O.grid(vc, gcf, N, nthreads=8)
     ref32 = ref.astype(np.complex64).astype(np.complex128)
     vref, _ = O.degrid(vc, gcf, ref32, nthreads=8)
-    g, v, st = run_tiled(vc, gcf, N, 64, 512, grid_in=ref32)
+    g, v, st = run_tiled(vc, gcf, N, tile, 512, grid_in=ref32, layout=layout)
     assert st["taps"] == taps and st["err"] == 0
     assert rel_l2(g, ref) <= TOL
     assert rel_l2(v, vref) <= TOL
