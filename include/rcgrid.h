@@ -371,6 +371,17 @@ rcg_status rcg_grid0(const rcg_double3 *uvw, const rcg_c128 *vis,
 /* count_scratch: grid_size^2 ints of device memory. */
 rcg_status rcg_reweight(const rcg_double3 *uvw, rcg_c128 *vis, double scale,
                         int n, int grid_size, int *count_scratch, void *stream);
+
+/* rcg_reweight without the count buffer argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops): no
+ * counterpart in the reference.  The counts go to a buffer of the library
+ * that each device holds for the life of the process (64 MiB), so the call
+ * allocates nothing; grid_size above RCG_REWEIGHT_STATIC_MAX_SIZE is refused
+ * with RCG_ERR_ARG.  Calls that overlap on one device share the buffer and
+ * race: serialise them (one stream, or events). */
+#define RCG_REWEIGHT_STATIC_MAX_SIZE 4096
+rcg_status rcg_reweight_static(const rcg_double3 *uvw, rcg_c128 *vis, double scale,
+                               int n, int grid_size, void *stream);
 rcg_status rcg_herm_padded(rcg_c128 *data, int size, int pitch, void *stream);
 /* herm_padded_inplace on complex128 (f64) or complex64 data. */
 rcg_status rcg_herm(void *data, int f64, int size, int pitch, void *stream);

@@ -96,6 +96,7 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_rotate": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, vp]),
     "rcg_grid0": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
     "rcg_reweight": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]),
+    "rcg_reweight_static": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp]),
     "rcg_herm_padded": (C.c_int, [vp, C.c_int, C.c_int, vp]),
     "rcg_herm": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "rcg_subtract_vis": (C.c_int, [vp, vp, C.c_int, C.c_longlong, vp]),

@@ -1,7 +1,8 @@
 // rcg_small.hip -- the small kernels next to the gridder: normalisation,
-// FP64 phase rotation, nearest-cell gridding, uniform reweighting, the
-// hermitian fold and the even-size fftshift; device-resident entry points
-// plus the reference's host-memory entry points over them.
+// FP64 phase rotation, nearest-cell gridding, the hermitian fold and the
+// even-size fftshift; device-resident entry points plus the reference's
+// host-memory entry points over them (uniform reweighting's device kernels
+// are in rcg_reweight.hip).
 //
 // Reference semantics (MS4/kernel/cpu/scatter_grid/
 // scatter_gridder_w_dependent_dyn_1p.cpp, MS4/kernel/cpu/herm/
@@ -60,47 +61,6 @@ __global__ void k_grid0(const rcg_double3 *__restrict__ uvw, const double2 *__re
     double *g = reinterpret_cast<double *>(centred + k);
     unsafeAtomicAdd(g, vis[i].x);
     unsafeAtomicAdd(g + 1, vis[i].y);
-  }
-}
-
-__global__ void k_zero_i32(int *__restrict__ p, long long n)
-{
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    p[i] = 0;
-}
-
-// Uniform weighting (.cpp:311-353): count per rounded cell, then divide.
-__device__ __forceinline__ long long reweight_cell(const rcg_double3 &p, double scale,
-                                                   int size, long long trans)
-{
-#pragma clang fp contract(off)
-  const long long u = (long long)round(p.u * scale);
-  const long long v = (long long)round(p.v * scale);
-  return u * size + v + trans;
-}
-
-__global__ void k_reweight_count(const rcg_double3 *__restrict__ uvw, int n, double scale,
-                                 int size, long long last, long long trans,
-                                 int *__restrict__ count)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long kg = reweight_cell(uvw[i], scale, size, trans);
-  if (kg >= 0 && kg < last) atomicAdd(count + kg, 1);
-}
-
-__global__ void k_reweight_apply(const rcg_double3 *__restrict__ uvw, double2 *__restrict__ vis,
-                                 int n, double scale, int size, long long last,
-                                 long long trans, const int *__restrict__ count)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long kg = reweight_cell(uvw[i], scale, size, trans);
-  if (kg >= 0 && kg < last) {
-    const double c = (double)count[kg];
-    vis[i].x /= c;
-    vis[i].y /= c;
   }
 }
 
@@ -325,29 +285,6 @@ extern "C" rcg_status rcg_grid0(const rcg_double3 *uvw, const rcg_c128 *vis,
                      reinterpret_cast<const double2 *>(vis),
                      reinterpret_cast<double2 *>(grid_centred), scale, n, last, trans,
                      grid_pitch);
-  RCG_HIP_TRY(hipGetLastError());
-  return RCG_OK;
-}
-
-extern "C" rcg_status rcg_reweight(const rcg_double3 *uvw, rcg_c128 *vis, double scale, int n,
-                                   int grid_size, int *count_scratch, void *stream)
-{
-  clear_error();
-  if (grid_size <= 0 || (long long)grid_size * grid_size > 0x7fffffffLL || n < 0 ||
-      (n > 0 && (!uvw || !vis || !count_scratch))) {
-    set_error(RCG_ERR_ARG, "rcg_reweight: bad arguments");
-    return RCG_ERR_ARG;
-  }
-  if (n == 0) return RCG_OK;
-  hipStream_t st = (hipStream_t)stream;
-  const long long last = (long long)grid_size * grid_size;
-  const long long trans = (long long)(grid_size / 2) * (grid_size + 1);
-  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(last)), dim3(256), 0, st, count_scratch, last);
-  hipLaunchKernelGGL(k_reweight_count, dim3((n + 255) / 256), dim3(256), 0, st, uvw, n, scale,
-                     grid_size, last, trans, count_scratch);
-  hipLaunchKernelGGL(k_reweight_apply, dim3((n + 255) / 256), dim3(256), 0, st, uvw,
-                     reinterpret_cast<double2 *>(vis), n, scale, grid_size, last, trans,
-                     count_scratch);
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
