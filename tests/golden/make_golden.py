@@ -19,6 +19,12 @@ Fixtures (inputs and expected outputs, .npz):
                       sampled points; sha256 of the generated inputs.
   small_kernels.npz   normalizeCPU, rotateCPU, grid0 and reweight on small
                       random inputs.
+  callers.npz         gridKernelCPUFullGCF and deGridKernelCPUFullGCF called
+                      with the tables and arguments of the reference's two
+                      callers (tests/callers.py): cppcycle's and the Haskell
+                      gridWrapper's (table pointer 32 entries past the centre).
+
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py callers   # one fixture
 """
 from __future__ import annotations
 
@@ -37,6 +43,9 @@ sys.path.insert(0, str(ROOT))
 from oracle import oracle as O  # noqa: E402
 from rc_amd import synth  # noqa: E402
 from rc_amd.tables import OVER, GCFTable, VisChunks  # noqa: E402
+
+sys.path.insert(0, str(HERE.parent))
+import callers as K  # noqa: E402
 
 
 def input_hash(vc: VisChunks, gcf: GCFTable) -> str:
@@ -198,21 +207,56 @@ def make_small(out: Path) -> None:
                         scale=scale, rot=rot, grid0=g0, reweight=rw)
 
 
-def main() -> int:
+def caller_cases():
+    """The calls of callers.npz: (name, Call) with inputs generated from
+    fixed seeds (the fixture keeps inputs and outputs)."""
+    # cppcycle: its constants (growth 16, min 3, max 128, pad 2, wstep 1e4),
+    # a 128^2 grid and its scale rule (most far points fall outside)
+    wstep = 10000.0
+    supp = K.support_rule(3, 16, 128)
+    uvw, vis = K.baselines(21, 24, 16, wstep, 2.4, lim=1.0, far=22.0)
+    lay = K.make_layers(K.cppcycle_max_plane(uvw, wstep), supp, np.random.default_rng(31))
+    cpp = K.cppcycle_call(uvw, vis, wstep, supp, 128, 2, lay)
+    # gridWrapper: GCFPar min 8, growth 8, max 40, a square 96^2 grid
+    wstep = 2.0
+    supp = K.support_rule(8, 8, 40)
+    uvw, vis = K.baselines(22, 24, 16, wstep, 2.4, lim=96 / 2 / 1.3 - 22)
+    lay = K.make_layers(K.haskell_max_plane(uvw, wstep), supp, np.random.default_rng(32))
+    hs = K.wrapper_call(uvw, vis, wstep, supp, 96, 96, 1.3, lay)
+    return [("cpp", cpp), ("hs", hs)]
+
+
+def make_callers(out: Path) -> None:
+    ref = O.ref_lib()
+    arrs = {}
+    for name, call in caller_cases():
+        g, taps = call.grid(ref.gridKernelCPUFullGCF)
+        v, dtaps = call.degrid(ref.deGridKernelCPUFullGCF, g)
+        arrs.update({f"{name}_uvw": call.uvw, f"{name}_vis": call.vis,
+                     f"{name}_chunk_supps": call.chunk_supps, f"{name}_grid": g,
+                     f"{name}_taps": taps, f"{name}_vis_out": v, f"{name}_dtaps": dtaps,
+                     f"{name}_layers_sha256": hashlib.sha256(call.layers.data.tobytes()).hexdigest()})
+    np.savez_compressed(out, **arrs)
+
+
+def main(argv: list[str]) -> int:
     if os.environ.get("OMP_NUM_THREADS") != "1":
         print("run with OMP_NUM_THREADS=1 (sequential reference summation order)")
         return 2
     if not O.ref_available():
         print("oracle/_ref/libms4ref.so missing: make -C oracle ref")
         return 2
-    make_probe(HERE / "pregrid_probe.npz")
-    make_random(HERE)
-    make_c1(HERE / "c1_summary.npz")
-    make_small(HERE / "small_kernels.npz")
+    makers = {"pregrid_probe": lambda: make_probe(HERE / "pregrid_probe.npz"),
+              "random": lambda: make_random(HERE),
+              "c1_summary": lambda: make_c1(HERE / "c1_summary.npz"),
+              "small_kernels": lambda: make_small(HERE / "small_kernels.npz"),
+              "callers": lambda: make_callers(HERE / "callers.npz")}
+    for name in (argv or list(makers)):
+        makers[name]()
     for p in sorted(HERE.glob("*.npz")):
         print(f"{p.name}: {p.stat().st_size} bytes")
     return 0
 
 
 if __name__ == "__main__":
-    raise SystemExit(main())
+    raise SystemExit(main(sys.argv[1:]))
