@@ -187,6 +187,16 @@ void rcg_bin_free(void *data);
 int rcg_last_error(void);
 const char *rcg_last_error_string(void);
 
+/* Which device kernel the calling thread's last gridKernelCPU*GCF or
+ * deGridKernelCPU*GCF call ran: RCG_PATH_TILED (binned into uv tiles),
+ * RCG_PATH_PLAIN (one wavefront per point, FP64 atomics), RCG_PATH_NONE
+ * (no kernel: an argument error or no in-bound point).  Diagnostics only;
+ * the reference has no counterpart. */
+#define RCG_PATH_NONE 0
+#define RCG_PATH_TILED 1
+#define RCG_PATH_PLAIN 2
+int rcg_legacy_path(void);
+
 /* ---------------------------------------------------------------------
  * 2. Device-resident API
  * ------------------------------------------------------------------- */

@@ -60,6 +60,7 @@ _lib: C.CDLL | None = None
 _LEGACY_GRID_ARGS = [C.c_double, C.c_double, C.c_int, vp, vp, vp, vp, vp,
                      C.c_int, C.c_int, C.c_int, vp]
 SIGNATURES: dict[str, tuple] = {
+    "rcg_legacy_path": (C.c_int, []),
     "gridKernelCPUFullGCF": (ull, _LEGACY_GRID_ARGS),
     "gridKernelCPUHalfGCF": (ull, _LEGACY_GRID_ARGS),
     "deGridKernelCPUFullGCF": (ull, _LEGACY_GRID_ARGS),

@@ -67,9 +67,14 @@ struct LegacyArgs {
   const int *gcf_supps;          // centred
 };
 
+// Gridder the calling thread's last gridKernelCPU*GCF call ran
+// (rcg_legacy_grid_path).
+thread_local int t_last_path = RCG_PATH_NONE;
+
 ull legacy_call(const LegacyArgs &a)
 {
   clear_error();
+  t_last_path = RCG_PATH_NONE;
   const size_t gcells = (a.size > 0 && a.pitch > 0) ? (size_t)a.size * a.pitch : 0;
   auto zero_outputs = [&]() {
     if (!a.degrid) {
@@ -282,18 +287,22 @@ ull legacy_call(const LegacyArgs &a)
     LEG_TRY(hipMemcpyAsync(d_vis.p, hvis, (size_t)n * sizeof(rcg_c128),
                            hipMemcpyHostToDevice, st));
     // The tiled FP64 gridder over binned pairs where a plan fits the grid
-    // (at most 16384 tiles) and the supports (at most 64 tiles per window);
-    // otherwise the plain FP64 gridder.
+    // (at most MAX_PLAN_TILES tiles: every grid of the int16 coordinate
+    // range at tile 32) and the supports (at most 64 tiles per window);
+    // otherwise, or when the plan's memory cannot be had, the plain FP64
+    // gridder.
     PlanGuard plan;
     int max_s = 1;
     for (int bl = 0; bl < a.baselines; bl++) max_s = std::max(max_s, a.bl_supps[bl]);
     for (int tile : {32, 64}) {
       const long long nt = (a.size + tile - 1) / tile;
-      if (nt * nt > 16384) continue;
+      if (nt * nt > MAX_PLAN_TILES) continue;
       if (rcg_tiled_plan_create(&g, tile, 1024, max_s, &plan.p) == RCG_OK) break;
       plan.p = nullptr;
     }
     clear_error();
+    (void)hipGetLastError();  // a failed plan allocation leaves its error behind
+    t_last_path = plan.p ? RCG_PATH_TILED : RCG_PATH_PLAIN;
     if (plan.p) {
       if (rcg_tiled_bin(plan.p, &t, st) != RCG_OK ||
           rcg_grid_tiled_f64(plan.p, &t, d_vis.as<rcg_c128>(), d_grid.as<rcg_c128>(), st) !=
@@ -387,3 +396,5 @@ RCG_LEGACY_GRID(gridKernelCPUFullGCF, false)
 RCG_LEGACY_GRID(gridKernelCPUHalfGCF, true)
 RCG_LEGACY_DEGRID(deGridKernelCPUFullGCF, false)
 RCG_LEGACY_DEGRID(deGridKernelCPUHalfGCF, true)
+
+extern "C" int rcg_legacy_path(void) { return rcg::t_last_path; }
