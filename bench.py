@@ -475,9 +475,10 @@ def bench_ms4(args) -> int:
     306 samples, each chunk's support that of its smallest-|w| plane,
     complex128 throughout.  One step = one grid call + one degrid call of the
     gridded result, host memory in and out (PCIe copies included).  Also
-    reported: the device-resident FP64 gridders on the same points, timed
-    with HIP events on the stream they run on: rcg_grid_tiled_f64 (binned
-    pairs, what the grid call runs) and the plain FP64 gridder."""
+    reported: the device-resident FP64 gridders and degridders on the same
+    points, timed with HIP events on the stream they run on:
+    rcg_grid_tiled_f64 / rcg_degrid_tiled_f64 (binned pairs, what the calls
+    run) and the plain FP64 kernels."""
     import torch
     from rc_amd import device as D
     from rc_amd import ms4, synth
@@ -518,9 +519,15 @@ def bench_ms4(args) -> int:
     ms_tiled = timed(lambda: plan.grid_f64(dg, dvis, gt), max(3, args.steps))
     ms_plain = timed(lambda: D.grid_plain_f64(geom, dg, pg, dvis, gp), 3)
     rel = float((gt - gp).abs().pow(2).sum().sqrt() / gp.abs().pow(2).sum().sqrt())
+    # the FP64 degridders on the gridded result
+    vt = plan.degrid_f64(dg, gp)
+    vpl = D.degrid_plain_f64(geom, dg, pg, gp)
+    ms_dtiled = timed(lambda: plan.degrid_f64(dg, gp, vt), max(3, args.steps))
+    ms_dplain = timed(lambda: D.degrid_plain_f64(geom, dg, pg, gp, vpl), 3)
+    drel = float((vt - vpl).abs().pow(2).sum().sqrt() / vpl.abs().pow(2).sum().sqrt())
     taps = int(taps_d.item())
     plan.close()
-    del gt, gp, pg, dvis, dg, geom
+    del gt, gp, pg, dvis, dg, geom, vt, vpl
     torch.cuda.empty_cache()
 
     # the host-memory calls
@@ -563,6 +570,9 @@ def bench_ms4(args) -> int:
                                 "speedup": round(ms_plain / ms_tiled, 2),
                                 "tiled_gtaps_per_s": round(taps / ms_tiled / 1e6, 1),
                                 "rel_l2_tiled_vs_plain": rel},
+        "device_fp64_degrid_ms": {"tiled": round(ms_dtiled, 4), "plain": round(ms_dplain, 4),
+                                  "speedup": round(ms_dplain / ms_dtiled, 2),
+                                  "rel_l2_tiled_vs_plain": drel},
     }
     if not args.no_cpu:
         from oracle import oracle as O

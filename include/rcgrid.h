@@ -293,7 +293,7 @@ rcg_status rcg_tiled_plan_create_batch(const rcg_geom *geoms, int n_inst,
 /* The same with a choice of record layout.  RCG_LAYOUT_PAIRS (the layout of
  * the two functions above): one record per (point, tile) that the point's
  * window overlaps, tile 16, 32 or 64; the only layout rcg_grid_tiled_f64
- * takes.  RCG_LAYOUT_HALO: one record per point whose support is at most 32
+ * and rcg_degrid_tiled_f64 take.  RCG_LAYOUT_HALO: one record per point whose support is at most 32
  * (16 when max_support <= 16), binned in the tile of its window's first
  * cell; larger windows are cut into ceil(S/32)^2 near-equal sub-windows,
  * each binned likewise; tile 16 or 32.  The kernels then hold a tile plus a
@@ -324,6 +324,14 @@ rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,
  * rcg_c128.  The gridder of the MS4 host-memory entry points. */
 rcg_status rcg_grid_tiled_f64(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                               const rcg_c128 *vis, rcg_c128 *grid, void *stream);
+/* FP64 degridding over a plan's pairs (binned as for rcg_grid_tiled_f64):
+ * each visibility is the sum of the reference's products grid * tap in FP64
+ * (scatter_gridder_w_dependent_dyn_1p.cpp:368-431, behind
+ * deGridKernelCPUFullGCF / HalfGCF) in another order; points left out get
+ * 0.  The degridder of the MS4 host-memory entry points. */
+rcg_status rcg_degrid_tiled_f64(rcg_tiled_plan *plan, const rcg_gcf *gcf,
+                                const rcg_c128 *grid, rcg_c128 *vis,
+                                void *stream);
 /* The Hermitian adjoint of rcg_degrid_tiled_f32: gridding with conjugated
  * taps (the reference's degridder is the plain transpose of its gridder,
  * scatter_gridder_w_dependent_dyn_1p.cpp:368-431, so the adjoint conjugates).  Used as the backward pass of a

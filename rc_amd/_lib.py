@@ -90,6 +90,7 @@ SIGNATURES: dict[str, tuple] = {
     "rcg_grid_tiled_adjoint_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_degrid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_grid_tiled_f64": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
+    "rcg_degrid_tiled_f64": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_tiled_stats": (C.c_int, [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_longlong),
                                   C.POINTER(C.c_longlong), C.POINTER(C.c_int), vp]),
     # small kernels (device)

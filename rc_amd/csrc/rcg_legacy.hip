@@ -8,10 +8,10 @@
 // buffer (or copy straight from the caller's memory when its chunks are
 // consecutive, as the reference's callers lay them out), find the w-plane
 // range of the in-bound points on the GPU, upload only the GCF layers of
-// those planes, pregrid (FP64, exact), grid (tiled FP64 gridder over binned
-// pairs, or the plain FP64 gridder where no plan fits) or degrid (plain FP64
-// degridder), copy the result back.  No state survives the call, as in the reference;
-// calls from different host threads use their own streams and buffers.
+// those planes, pregrid (FP64, exact), grid or degrid (tiled FP64 kernels
+// over binned pairs, or the plain FP64 ones where no plan fits), copy the
+// result back.  No state survives the call, as in the reference; calls from
+// different host threads use their own streams and buffers.
 #include <algorithm>
 #include <climits>
 #include <cstring>
@@ -270,6 +270,32 @@ ull legacy_call(const LegacyArgs &a)
   DevBuf d_vis, d_grid;
   LEG_TRY(d_vis.alloc((size_t)n * sizeof(rcg_c128)));
   LEG_TRY(d_grid.alloc(gcells * sizeof(rcg_c128)));
+  // The tiled FP64 gridder and degridder over binned pairs where a plan fits
+  // the grid (at most MAX_PLAN_TILES tiles: every grid of the int16
+  // coordinate range at tile 32) and the supports (at most 64 tiles per
+  // window); otherwise, or when the plan's memory cannot be had, the plain
+  // FP64 kernels.
+  PlanGuard plan;
+  int max_s = 1;
+  for (int bl = 0; bl < a.baselines; bl++) max_s = std::max(max_s, a.bl_supps[bl]);
+  for (int tile : {32, 64}) {
+    const long long nt = (a.size + tile - 1) / tile;
+    if (nt * nt > MAX_PLAN_TILES) continue;
+    if (rcg_tiled_plan_create(&g, tile, 1024, max_s, &plan.p) == RCG_OK) break;
+    plan.p = nullptr;
+  }
+  clear_error();
+  (void)hipGetLastError();  // a failed plan allocation leaves its error behind
+  if (plan.p) {
+    if (rcg_tiled_bin(plan.p, &t, st) != RCG_OK) return fail();
+    int perr = 0;
+    if (rcg_tiled_stats(plan.p, nullptr, nullptr, nullptr, &perr, st) != RCG_OK) return fail();
+    if (perr & ERR_CAPACITY) {
+      set_error(RCG_ERR_HIP, "tiled plan: pair buffer overflow");
+      return fail();
+    }
+  }
+  t_last_path = plan.p ? RCG_PATH_TILED : RCG_PATH_PLAIN;
   if (!a.degrid) {
     std::unique_ptr<rcg_c128[]> hvis_buf;
     const rcg_c128 *hvis = a.vis_in[0];
@@ -286,34 +312,10 @@ ull legacy_call(const LegacyArgs &a)
     }
     LEG_TRY(hipMemcpyAsync(d_vis.p, hvis, (size_t)n * sizeof(rcg_c128),
                            hipMemcpyHostToDevice, st));
-    // The tiled FP64 gridder over binned pairs where a plan fits the grid
-    // (at most MAX_PLAN_TILES tiles: every grid of the int16 coordinate
-    // range at tile 32) and the supports (at most 64 tiles per window);
-    // otherwise, or when the plan's memory cannot be had, the plain FP64
-    // gridder.
-    PlanGuard plan;
-    int max_s = 1;
-    for (int bl = 0; bl < a.baselines; bl++) max_s = std::max(max_s, a.bl_supps[bl]);
-    for (int tile : {32, 64}) {
-      const long long nt = (a.size + tile - 1) / tile;
-      if (nt * nt > MAX_PLAN_TILES) continue;
-      if (rcg_tiled_plan_create(&g, tile, 1024, max_s, &plan.p) == RCG_OK) break;
-      plan.p = nullptr;
-    }
-    clear_error();
-    (void)hipGetLastError();  // a failed plan allocation leaves its error behind
-    t_last_path = plan.p ? RCG_PATH_TILED : RCG_PATH_PLAIN;
     if (plan.p) {
-      if (rcg_tiled_bin(plan.p, &t, st) != RCG_OK ||
-          rcg_grid_tiled_f64(plan.p, &t, d_vis.as<rcg_c128>(), d_grid.as<rcg_c128>(), st) !=
-              RCG_OK)
+      if (rcg_grid_tiled_f64(plan.p, &t, d_vis.as<rcg_c128>(), d_grid.as<rcg_c128>(), st) !=
+          RCG_OK)
         return fail();
-      int perr = 0;
-      if (rcg_tiled_stats(plan.p, nullptr, nullptr, nullptr, &perr, st) != RCG_OK) return fail();
-      if (perr & ERR_CAPACITY) {
-        set_error(RCG_ERR_HIP, "gridding plan: pair buffer overflow");
-        return fail();
-      }
     } else if (rcg_grid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_vis.as<rcg_c128>(),
                                   d_grid.as<rcg_c128>(), st) != RCG_OK) {
       return fail();
@@ -324,8 +326,10 @@ ull legacy_call(const LegacyArgs &a)
   } else {
     LEG_TRY(hipMemcpyAsync(d_grid.p, a.grid_in, gcells * sizeof(rcg_c128),
                            hipMemcpyHostToDevice, st));
-    if (rcg_degrid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_grid.as<rcg_c128>(),
-                             d_vis.as<rcg_c128>(), st) != RCG_OK)
+    if (plan.p ? rcg_degrid_tiled_f64(plan.p, &t, d_grid.as<rcg_c128>(),
+                                      d_vis.as<rcg_c128>(), st) != RCG_OK
+               : rcg_degrid_plain_f64(&g, &t, d_pg.as<rcg_pg>(), d_grid.as<rcg_c128>(),
+                                      d_vis.as<rcg_c128>(), st) != RCG_OK)
       return fail();
     if (consecutive(reinterpret_cast<const void *const *>(a.vis_out), sizeof(rcg_c128))) {
       LEG_TRY(hipMemcpyAsync(a.vis_out[0], d_vis.p, (size_t)n * sizeof(rcg_c128),

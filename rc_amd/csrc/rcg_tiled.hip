@@ -724,6 +724,78 @@ k_grid_tiles_f64(TiledDev d, const double2 *__restrict__ vis,
   });
 }
 
+// FP64 degridding over the same pairs (the degridder of the MS4 host-memory
+// entry points, reference scatter_gridder_w_dependent_dyn_1p.cpp:368-431):
+// the workgroup stages the tile of the grid in LDS; each wavefront takes
+// pairs in turn, its lanes covering the window/tile intersection as in
+// k_grid_tiles_f64 and summing the reference's complex products
+// grid * tap (conjugated taps for mirrored layers) in FP64 without
+// contraction; the 64 lane sums are added across the wavefront.  A pair
+// holding a point's whole window stores its visibility; the pairs of a
+// window split over tiles add theirs with FP64 atomics into the zeroed
+// output (only the order of the additions differs from the reference).
+template <int T>
+__global__ void __launch_bounds__(64 * F64_WAVES)
+k_degrid_tiles_f64(TiledDev d, const double2 *__restrict__ grid,
+                   const double2 *__restrict__ taps, long long n_taps,
+                   double2 *__restrict__ vis)
+{
+#pragma clang fp contract(off)
+  constexpr int LS = T + 1;
+  __shared__ double2 tile[T * LS];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for_each_item(d, d.queue + 8, [&](const ItemRange &ir) {
+    const int gsz = d.grid_size;
+    const double2 *gi = grid + ir.goff;
+    for (int q = threadIdx.x; q < T * T; q += 64 * F64_WAVES) {
+      const int r = q / T, c = q - (q / T) * T;
+      const int gu = ir.tu0 + r, gv = ir.tv0 + c;
+      tile[r * LS + c] = (gu < gsz && gv < gsz) ? gi[(long long)gu * d.pitch + gv]
+                                                : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int k = ir.begin + wave; k < ir.end; k += F64_WAVES) {
+      const int4 rec = d.pairs[k];
+      const int r0 = rec.x & 63, c0 = (rec.x >> 6) & 63;
+      const int w = ((rec.x >> 12) & 63) + 1;
+      const int rows = ((rec.x >> 18) & 63) + 1;
+      const int stride = rec.z & 0xffff, h = (rec.z >> 16) & 127;
+      const int lr = lane / w, lc = lane - lr * w;
+      double sx = 0.0, sy = 0.0;
+      if (lr < rows) {
+        const double sgn = (rec.x >> 30) & 1 ? -1.0 : 1.0;
+        long long ti = (long long)rec.y + (long long)lr * stride + lc;
+        const long long tstep = (long long)rows * stride;
+        const double2 *cell = tile + (r0 + lr) * LS + c0 + lc;
+        const int cstep = rows * LS;
+        for (int row = lr; row < h; row += rows, ti += tstep, cell += cstep) {
+          if (ti < 0 || ti >= n_taps) continue;
+          double2 t = taps[ti];
+          t.y *= sgn;
+          const double2 g = *cell;
+          sx += g.x * t.x - g.y * t.y;
+          sy += g.x * t.y + g.y * t.x;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off);
+        sy += __shfl_xor(sy, off);
+      }
+      if (lane == 0) {
+        double2 *o = vis + rec.w;
+        if (rec.x < 0) {  // the whole window lies in this tile
+          *o = make_double2(sx, sy);
+        } else {
+          unsafeAtomicAdd(&o->x, sx);
+          unsafeAtomicAdd(&o->y, sy);
+        }
+      }
+    }
+  });
+}
+
 // grid += lo, cell by cell (the end of a compensated gridding)
 __global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restrict__ lo,
                                long long n)
@@ -1447,7 +1519,69 @@ rcg_status grid_f64(rcg_tiled_plan *pl, const double2 *v, const double2 *t, long
   return RCG_OK;
 }
 
+template <int T>
+rcg_status launch_degrid_f64(const TiledDev &d, const double2 *g, const double2 *t,
+                             long long nt, double2 *v, hipStream_t st)
+{
+  int nb = 0;
+  RCG_HIP_TRY(persistent_blocks(reinterpret_cast<const void *>(&k_degrid_tiles_f64<T>),
+                                64 * F64_WAVES, &nb));
+  hipLaunchKernelGGL(k_degrid_tiles_f64<T>, dim3(nb), dim3(64 * F64_WAVES), 0, st, d, g, t, nt,
+                     v);
+  RCG_HIP_TRY(hipGetLastError());
+  return RCG_OK;
+}
+
+rcg_status degrid_f64(rcg_tiled_plan *pl, const double2 *g, const double2 *t, long long nt,
+                      double2 *v, hipStream_t st)
+{
+  TiledDev d = pl->dev;
+  if (d.layout != LAYOUT_PAIRS) {
+    set_error(RCG_ERR_UNSUPPORTED, "FP64 tiled degridding needs a plan of RCG_LAYOUT_PAIRS");
+    return RCG_ERR_UNSUPPORTED;
+  }
+  // zeroed output: points left out stay 0, split windows add into it
+  hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(256), 0, st, reinterpret_cast<float *>(v),
+                     d.npts * 4);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d.queue + 8, 0, 8LL);
+  RCG_HIP_TRY(hipGetLastError());
+  if (pl->n == 0) return RCG_OK;
+  hipEvent_t *ev = timing_slot(pl, 1);
+  if (ev) RCG_HIP_TRY(hipEventRecord(ev[0], st));
+  rcg_status s;
+  switch (d.T) {
+    case 16: s = launch_degrid_f64<16>(d, g, t, nt, v, st); break;
+    case 32: s = launch_degrid_f64<32>(d, g, t, nt, v, st); break;
+    default: s = launch_degrid_f64<64>(d, g, t, nt, v, st); break;
+  }
+  if (s != RCG_OK) return s;
+  if (ev) {
+    RCG_HIP_TRY(hipEventRecord(ev[1], st));
+    pl->n_ev[1]++;
+  }
+  return RCG_OK;
+}
+
 }  // namespace
+
+extern "C" rcg_status rcg_degrid_tiled_f64(rcg_tiled_plan *pl, const rcg_gcf *gcf,
+                                           const rcg_c128 *grid, rcg_c128 *vis, void *stream)
+{
+  clear_error();
+  if (!pl || !gcf || !vis || !grid) {
+    set_error(RCG_ERR_ARG, "rcg_degrid_tiled_f64: null argument");
+    return RCG_ERR_ARG;
+  }
+  rcg_status s = check_gcf(gcf);
+  if (s != RCG_OK) return s;
+  if (gcf->n_taps <= 0) {
+    set_error(RCG_ERR_ARG, "rcg_degrid_tiled_f64: empty tap table");
+    return RCG_ERR_ARG;
+  }
+  return degrid_f64(pl, reinterpret_cast<const double2 *>(grid),
+                    reinterpret_cast<const double2 *>(gcf->data), gcf->n_taps,
+                    reinterpret_cast<double2 *>(vis), (hipStream_t)stream);
+}
 
 extern "C" rcg_status rcg_grid_tiled_f64(rcg_tiled_plan *pl, const rcg_gcf *gcf,
                                          const rcg_c128 *vis, rcg_c128 *grid, void *stream)

@@ -234,6 +234,25 @@ class TiledPlan(_lib.Handle):
                    "rcg_grid_tiled_f64")
         return grid
 
+    def degrid_f64(self, gcf: DeviceGCF, grid: torch.Tensor, vis: torch.Tensor | None = None,
+                   stream=None) -> torch.Tensor:
+        """FP64 degridding over the plan's pairs (rcg_degrid_tiled_f64):
+        complex128 taps, grid and visibilities."""
+        if gcf.data.dtype != torch.complex128:
+            raise TypeError("FP64 degridding takes complex128 taps")
+        if grid.dtype != torch.complex128 or tuple(grid.shape) != self.grid_shape or \
+                not grid.is_contiguous():
+            raise ValueError(f"grid must be a contiguous complex128 {self.grid_shape} tensor")
+        if vis is None:
+            vis = torch.empty(self.n, dtype=torch.complex128, device=grid.device)
+        if vis.dtype != torch.complex128 or vis.numel() != self.n or not vis.is_contiguous():
+            raise ValueError("vis must be a contiguous complex128 tensor of n points")
+        t = gcf.struct()
+        _lib.check(_lib.lib().rcg_degrid_tiled_f64(self._h, C.byref(t), grid.data_ptr(),
+                                                   vis.data_ptr(), _stream(stream)),
+                   "rcg_degrid_tiled_f64")
+        return vis
+
     def degrid(self, gcf: DeviceGCF, grid: torch.Tensor, vis: torch.Tensor | None = None,
                stream=None) -> torch.Tensor:
         if gcf.data.dtype != torch.complex64:

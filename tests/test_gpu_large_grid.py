@@ -4,8 +4,9 @@ sparse w-projection sample (64 antennas x 64 times = 129,024 points, 16
 w-planes, supports 8..64, half table), against the oracle on the same
 complex64-rounded inputs: the FP32 tiled path in both record layouts (grid
 and degrid, relative L2 <= 1e-5), the FP64 tiled gridder (<= 1e-12), error
-word 0 and tap counts equal; and the MS4 host call gridKernelCPUHalfGCF at
-that size runs the tiled FP64 gridder (not the plain one) within 1e-12.
+word 0 and tap counts equal, the FP64 tiled degridder (<= 1e-12); and the
+MS4 host calls gridKernelCPUHalfGCF / deGridKernelCPUHalfGCF at that size
+run the tiled FP64 kernels (not the plain ones) within 1e-12.
 The reference takes any grid its int16 coordinates allow
 (MS4/kernel/common/common.h:128-134)."""
 import numpy as np
@@ -72,9 +73,26 @@ def test_fp64_tiled_16384(hip_device, case):
     assert rel_l2(g.cpu().numpy(), ref) <= 1e-12
 
 
-def test_ms4_host_call_16384_runs_tiled(hip_device, case):
+def test_fp64_tiled_degrid_16384(hip_device, case):
+    vc, gcf, ref, taps = case
+    geom = D.DeviceGeom.from_chunks(vc, N)
+    dg = D.DeviceGCF.from_table(gcf, dtype=np.complex128)
+    plan = D.TiledPlan(geom, tile=32, chunk=1024, max_support=64)
+    plan.bin(dg)
+    v = plan.degrid_f64(dg, torch.from_numpy(ref).cuda()).cpu().numpy()
+    plan.close()
+    vref, _ = O.degrid(vc, gcf, ref, nthreads=16)
+    assert rel_l2(v, vref.reshape(-1)) <= 1e-12
+
+
+def test_ms4_host_calls_16384_run_tiled(hip_device, case):
     vc, gcf, ref, taps = case
     g, t = ms4.grid(vc, gcf, N)
     assert _lib.lib().rcg_legacy_path() == RCG_PATH_TILED
     assert t == taps
     assert rel_l2(g, ref) <= 1e-12
+    v, dt = ms4.degrid(vc, gcf, ref)
+    assert _lib.lib().rcg_legacy_path() == RCG_PATH_TILED
+    vref, _ = O.degrid(vc, gcf, ref, nthreads=16)
+    assert dt == taps
+    assert rel_l2(v, vref) <= 1e-12

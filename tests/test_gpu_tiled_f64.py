@@ -1,9 +1,10 @@
-"""GPU parity of rcg_grid_tiled_f64 (FP64 gridding over a tiled plan's pairs,
-the gridder of the MS4 host-memory entry points) against the FP64 oracle.
-Every tap adds the reference's complex product in FP64
-(MS4/kernel/cpu/scatter_grid/scatter_gridder_w_dependent_dyn_1p.cpp:169);
-only the summation order differs, so the bar is the MS4 one: tap counts
-equal, relative L2 <= 1e-12."""
+"""GPU parity of rcg_grid_tiled_f64 and rcg_degrid_tiled_f64 (FP64 gridding
+and degridding over a tiled plan's pairs, the kernels of the MS4
+host-memory entry points) against the FP64 oracle.  Every tap adds the
+reference's complex product in FP64
+(MS4/kernel/cpu/scatter_grid/scatter_gridder_w_dependent_dyn_1p.cpp:169 and
+:368-431); only the summation order differs, so the bar is the MS4 one: tap
+counts equal, relative L2 <= 1e-12."""
 import numpy as np
 import pytest
 import torch
@@ -18,7 +19,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def run_f64(vc, gcf, N, tile, chunk, pitch=None):
+def run_f64(vc, gcf, N, tile, chunk, pitch=None, grid_in=None):
+    """(grid, stats) of rcg_grid_tiled_f64, and with grid_in the
+    visibilities rcg_degrid_tiled_f64 takes from it."""
     geom = D.DeviceGeom.from_chunks(vc, N, pitch)
     dg = D.DeviceGCF.from_table(gcf)
     plan = D.TiledPlan(geom, tile=tile, chunk=chunk)
@@ -26,10 +29,14 @@ def run_f64(vc, gcf, N, tile, chunk, pitch=None):
     vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex128)).cuda()
     g = plan.grid_f64(dg, vis)
     st = plan.stats()
+    v = None
+    if grid_in is not None:
+        v = plan.degrid_f64(dg, torch.from_numpy(np.ascontiguousarray(grid_in)).cuda())
+        v = v.cpu().numpy().reshape(vc.n_chunks, vc.ts_ch)
     torch.cuda.synchronize()
     out = g.cpu().numpy(), st
     plan.close()
-    return out
+    return out if grid_in is None else out + (v,)
 
 
 @pytest.mark.parametrize("tile", [16, 32, 64])
@@ -41,9 +48,12 @@ def test_tiled_f64_random_vs_oracle(hip_device, tile, chunk, seed, half, cs):
                                     half=half, chunk_supp=cs, supports=(6, 17, 40))
     N = 160
     ref, taps = O.grid(vc, gcf, N, nthreads=4)
-    g, st = run_f64(vc, gcf, N, tile, chunk)
-    assert st["err"] == 0 and st["taps"] == taps
+    vref, dtaps = O.degrid(vc, gcf, ref, nthreads=4)
+    g, st, v = run_f64(vc, gcf, N, tile, chunk, grid_in=ref)
+    assert st["err"] == 0 and st["taps"] == taps == dtaps
     assert rel_l2(g, ref) <= TOL
+    assert rel_l2(v, vref) <= TOL
+    assert np.array_equal(v == 0, vref == 0)  # points left out are 0
 
 
 def test_tiled_f64_padded_pitch_large_supports(hip_device):
@@ -52,9 +62,11 @@ def test_tiled_f64_padded_pitch_large_supports(hip_device):
     vc, gcf, N = synth.random_small(21, grid_size=320, chunks=6, ts_ch=7, max_plane=2,
                                     half=True, supports=(64, 72, 80), chunk_supp="random")
     ref, taps = O.grid(vc, gcf, N, N + 5)
-    g, st = run_f64(vc, gcf, N, 32, 64, N + 5)
+    vref, _ = O.degrid(vc, gcf, ref)
+    g, st, v = run_f64(vc, gcf, N, 32, 64, N + 5, grid_in=ref)
     assert st["err"] == 0 and st["taps"] == taps
     assert rel_l2(g, ref) <= TOL
+    assert rel_l2(v, vref) <= TOL
 
 
 def test_tiled_f64_wproj_slice_vs_oracle(hip_device):
@@ -62,9 +74,11 @@ def test_tiled_f64_wproj_slice_vs_oracle(hip_device):
     with 32 w-planes (supports 8..64), complex128 throughout."""
     vc, gcf, N = synth.wproj(n_ant=64, n_times=64, vis_dtype=np.complex128)
     ref, taps = O.grid(vc, gcf, N, nthreads=8)
-    g, st = run_f64(vc, gcf, N, 32, 1024)
+    vref, _ = O.degrid(vc, gcf, ref, nthreads=8)
+    g, st, v = run_f64(vc, gcf, N, 32, 1024, grid_in=ref)
     assert st["err"] == 0 and st["taps"] == taps
     assert rel_l2(g, ref) <= TOL
+    assert rel_l2(v, vref) <= TOL
 
 
 def test_tiled_f64_type_checks(hip_device):
