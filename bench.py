@@ -29,7 +29,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD published
-FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
+# MI355X FP32 vector peak without packed FP32 instructions (which this
+# library does not use): 256 CUs x 64 lanes x 2 flops (FMA) x 2.4 GHz.  AMD's
+# 157.3 TFLOP/s counts packed FP32.
+FP32_VALU_PEAK_TFLOPS = 78.6
+CLOCK_GHZ = 2.4                # MI355X peak engine clock
+N_CU = 256
 
 
 def parse():
@@ -39,6 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tile", type=int, default=32)
     ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--waves", type=int, default=0,
+                    help="wavefronts per gridding workgroup, 1..4 (0: the plan's default)")
     ap.add_argument("--layout", choices=["pairs", "halo", "halo16"], default="pairs",
                     help="record layout of the tiled plan (see rcg_tiled_plan_create_ex)")
     ap.add_argument("--n-times", type=int, default=306)
@@ -46,7 +53,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", choices=["wproj", "batched", "imaging", "precision", "binner",
-                                           "flow", "small", "next", "ms4"],
+                                           "flow", "small", "next", "ms4", "tune"],
                     default="wproj",
                     help="wproj: the 4096^2 w-projection workload (default); batched: 64 "
                          "uneven 2048^2 snapshot instances, one graph launch vs one launch "
@@ -59,7 +66,8 @@ def parse():
                          "reweight, hermitian fold and FFT against their rooflines; next: "
                          "GCF generation, MS6 scatter/degrid and nvGridder; ms4: the MS4 "
                          "host-memory grid/degrid calls on 32640 chunks of 306 and the "
-                         "device FP64 gridders")
+                         "device FP64 gridders; tune: the autotuner's (tile, chunk, waves) "
+                         "sweep on the wproj workload")
     ap.add_argument("--instances", type=int, default=64)
     return ap.parse_args()
 
@@ -68,18 +76,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(kernel: str) -> int | None:
-    """HBM bytes per dispatch of `kernel` at the default workload, from the
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed under profiles/
-    (lines 'traffic_bytes <kernel> <bytes>'), or None."""
+def profiled(kernel: str) -> dict:
+    """Counter figures per dispatch of `kernel` at the default headline
+    workload, from the rocprofv3 summary committed under profiles/ (lines
+    '<key> <kernel> <value>': traffic_bytes = L2-to-memory bytes from the
+    per-request-size counters, valu_instr, salu_instr)."""
     p = ROOT / "profiles" / "wproj_tiled_traffic.md"
-    if not p.exists():
-        return None
-    for line in p.read_text().splitlines():
-        f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
-            return int(f[2])
-    return None
+    out = {}
+    if p.exists():
+        for line in p.read_text().splitlines():
+            f = line.split()
+            if len(f) == 3 and f[1] == kernel and f[0] in ("traffic_bytes", "valu_instr",
+                                                            "salu_instr"):
+                out[f[0]] = int(f[2])
+    return out
 
 
 def cpu_baseline(vc, gcf, N, budget_s: float) -> dict:
@@ -121,8 +131,8 @@ def cpu_baseline(vc, gcf, N, budget_s: float) -> dict:
 
 def bench_batched(args) -> int:
     """64 snapshot instances (2048^2 grids, 16 w-planes, supports 8..48,
-    lognormal point counts, median 400k, clipped to 20k..5M): tile and chunk
-    from the autotuner, then bin + grid + degrid of all instances as one
+    lognormal point counts, median 400k, clipped to 20k..5M): tile, chunk and
+    gridding waves per workgroup from the autotuner, then bin + grid + degrid of all instances as one
     captured graph (one launch per kernel over all instances) against one
     plan and one launch sequence per instance."""
     import torch
@@ -140,9 +150,9 @@ def bench_batched(args) -> int:
     vis = torch.from_numpy(np.concatenate([v.vis.reshape(-1) for v in vcs])
                            .astype(np.complex64)).to(dev)
     tune = autotune.autotune(geoms, dg, vis, max_support=48, log=log)
-    tile, chunk = tune["tile"], tune["chunk"]
-    log(f"autotune: tile {tile} chunk {chunk} ({tune['tune_s']:.1f} s)")
-    plan = D.TiledPlan(geoms, tile=tile, chunk=chunk, max_support=48)
+    tile, chunk, waves = tune["tile"], tune["chunk"], tune["waves"]
+    log(f"autotune: tile {tile} chunk {chunk} waves {waves} ({tune['tune_s']:.1f} s)")
+    plan = D.TiledPlan(geoms, tile=tile, chunk=chunk, max_support=48, waves=waves)
     grid = torch.empty(plan.grid_shape, dtype=torch.complex64, device=dev)
     out = torch.empty(n, dtype=torch.complex64, device=dev)
     graph = plan.capture(dg, D.STAGE_BIN | D.STAGE_GRID | D.STAGE_DEGRID, vis=vis, grid=grid,
@@ -165,7 +175,8 @@ def bench_batched(args) -> int:
         return 1
     # one plan and one launch sequence per instance, same tile and chunk
     offs = np.cumsum([0] + [g.n for g in geoms])
-    plans = [D.TiledPlan(g, tile=tile, chunk=chunk, max_support=48) for g in geoms]
+    plans = [D.TiledPlan(g, tile=tile, chunk=chunk, max_support=48, waves=waves)
+             for g in geoms]
 
     def per_instance():
         for k, p in enumerate(plans):
@@ -215,12 +226,13 @@ def bench_batched(args) -> int:
                                "supports 8-48, lognormal counts (median 400k, sigma 1, "
                                "20k-5M)",
                    "points": n, "taps_per_direction": st["taps"], "tile": tile,
-                   "chunk": chunk},
+                   "chunk": chunk, "waves": waves},
         "per_instance_ms_per_step": round(ms_single, 4),
         "per_instance_graphs_ms_per_step": round(ms_single_graph, 4),
         "speedup_vs_per_instance": round(ms_single / ms_batch, 3),
         "error_vs_fp64": {"grid_rel_l2": g_err, "degrid_rel_l2": d_err},
-        "autotune": [{k: r.get(k) for k in ("tile", "chunk", "ms_step", "skipped")}
+        "autotune": [{k: r.get(k) for k in ("tile", "chunk", "waves", "ms_step", "ms_grid",
+                                            "ms_degrid", "skipped")}
                      for r in tune["table"]],
     }
     print(json.dumps(line), flush=True)
@@ -902,6 +914,31 @@ def bench_next(args) -> int:
     return 0
 
 
+def bench_tune(args) -> int:
+    """The autotuner's sweep on the wproj workload (4096^2, 32 w-planes,
+    9,987,840 points): every (tile, chunk, waves) candidate timed over bin +
+    grid + degrid with the library's kernel events; prints the table and the
+    chosen point."""
+    import torch
+    from rc_amd import autotune
+    from rc_amd import device as D
+    from rc_amd import synth
+
+    vc, gcf, N = synth.wproj(n_times=args.n_times)
+    geom = D.DeviceGeom.from_chunks(vc, N)
+    dg = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
+    vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).cuda()
+    tune = autotune.autotune(geom, dg, vis, max_support=64, reps=max(2, args.steps), log=log)
+    line = {"workload": "wproj 4096x4096, 32 w-planes, supports 8-64", "points": vc.n,
+            "best": {k: tune[k] for k in ("tile", "chunk", "waves", "ms_step")},
+            "tune_s": round(tune["tune_s"], 1),
+            "table": [{k: (round(r[k], 4) if isinstance(r.get(k), float) else r.get(k))
+                       for k in ("tile", "chunk", "waves", "ms_step", "ms_grid", "ms_degrid",
+                                 "skipped")} for r in tune["table"]]}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -932,6 +969,8 @@ def main() -> int:
         return bench_next(args)
     if args.workload == "ms4":
         return bench_ms4(args)
+    if args.workload == "tune":
+        return bench_tune(args)
     dev = torch.device("cuda:0")
     t0 = time.time()
     vc, gcf, N = synth.wproj(n_times=args.n_times)
@@ -940,7 +979,7 @@ def main() -> int:
     dg32 = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
     vis32 = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).to(dev)
     plan = D.TiledPlan(geom, tile=args.tile, chunk=args.chunk, max_support=64,
-                       layout=args.layout)
+                       layout=args.layout, waves=args.waves)
     grid32 = torch.empty((N, N), dtype=torch.complex64, device=dev)
     out32 = torch.empty(vc.n, dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream()
@@ -1023,28 +1062,57 @@ def main() -> int:
                  "speedup_vs_fp64_plain": round(value / v64, 3),
                  "error_vs_fp64": {"grid_rel_l2": g_err, "degrid_rel_l2": d_err}}
 
-    # roofline of the dominant kernel (algorithmic bytes per launch: points x
-    # (24 B uvw + 8 B vis) + 8 B per tap + 8 B per grid cell), timed by the
-    # HIP events the library records around that kernel on its stream
-    b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
+    # Roofline of the dominant kernel, timed by the HIP events the library
+    # records around it on its stream.  Its bound is the larger of two
+    # fractions: bytes between L2 and memory (rocprofv3 per-request-size
+    # counters, profiled at the default configuration) against HBM peak, and
+    # FP32 work (8 flops per tap: the reference's accounting) against the
+    # unpacked FP32 peak.  Also reported: VALU issue (the useful FMA
+    # instructions, 4 per 64 taps, and all VALU/SALU instructions the
+    # counters saw, against one instruction per CU per cycle at peak clock),
+    # and the tap stream (8 B per tap read, mostly served from L2 and the
+    # MALL, so never an HBM figure).
     t_dom = max(k_grid, k_degrid)
     dom = "k_grid_tiles" if k_grid >= k_degrid else "k_degrid_tiles"
-    achieved = b_dir / (t_dom / 1e3) / 1e9
-    roof = {"bound": "hbm", "kernel": f"{dom}<{args.tile}>", "kernel_ms": round(t_dom, 4),
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "traffic_source": None,
-            "valu_tflops": round(8 * taps / (t_dom / 1e3) / 1e12, 3),
-            "valu_frac": round(8 * taps / (t_dom / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)}
-
-    # measured HBM bytes of this kernel (profiled at tile 32, chunk 1024 on the
-    # default workload only)
-    if args.tile == 32 and args.chunk == 1024 and args.n_times == 306:
-        tb = profiled_traffic(f"{dom}<{args.tile}>")
-        if tb is not None:
-            roof["traffic"] = tb
-            roof["traffic_source"] = ("profiles/wproj_tiled_traffic.md: rocprofv3 "
-                                      "(2*FETCH_SIZE + WRITE_SIZE) KiB per dispatch")
+    kname = f"{dom}<{args.tile}>"
+    sec = t_dom / 1e3
+    slots = sec * CLOCK_GHZ * 1e9 * N_CU
+    fp32_tf = 8 * taps / sec / 1e12
+    b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
+    default_cfg = (args.tile == 32 and args.chunk == 1024 and args.n_times == 306 and
+                   args.layout == "pairs" and args.waves == 0)
+    prof = profiled(kname) if default_cfg else {}
+    tb = prof.get("traffic_bytes")
+    hbm_gbs = tb / sec / 1e9 if tb else None
+    fracs = {"hbm": hbm_gbs / HBM_PEAK_GBS if tb else -1.0,
+             "fp32": fp32_tf / FP32_VALU_PEAK_TFLOPS}
+    bound = max(fracs, key=fracs.get)
+    if bound == "hbm":
+        roof = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(fracs["hbm"], 4)}
+    else:
+        roof = {"bound": "fp32", "achieved": round(fp32_tf, 3), "peak": FP32_VALU_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(fracs["fp32"], 4)}
+    roof.update({
+        "traffic": tb,
+        "traffic_source": ("profiles/wproj_tiled_traffic.md: rocprofv3 TCC_EA0_RDREQ_32B/64B/128B "
+                           "x 32/64/128 B + TCC_EA0_WRREQ(_64B) x 32/64 B per dispatch "
+                           "(L2 to memory, an upper bound of HBM bytes)") if tb else None,
+        "kernel": kname, "kernel_ms": round(t_dom, 4),
+        "fp32": {"achieved": round(fp32_tf, 3), "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(fracs["fp32"], 4), "flops_per_tap": 8},
+        "valu_issue": {"useful_instr": int(taps * 4 // 64), "slots": int(slots),
+                       "frac_useful": round(taps * 4 / 64 / slots, 4),
+                       "valu_instr": prof.get("valu_instr"), "salu_instr": prof.get("salu_instr"),
+                       "frac_valu": round(prof["valu_instr"] / slots, 4)
+                       if "valu_instr" in prof else None,
+                       "frac_salu": round(prof["salu_instr"] / slots, 4)
+                       if "salu_instr" in prof else None,
+                       "clock_ghz": CLOCK_GHZ},
+        "tap_stream": {"bytes": b_dir, "gbs": round(b_dir / sec / 1e9, 1),
+                       "note": "uvw + vis + 8 B per tap + grid per direction; the taps are "
+                               "re-read from L2/MALL, so this is not an HBM figure"},
+    })
     cpu = None
     if not args.no_cpu:
         try:
@@ -1062,7 +1130,8 @@ def main() -> int:
         "config": {"workload": "w-projection grid+degrid, 4096x4096, 32 w-planes, "
                                "supports 8-64, 8x oversampling",
                    "points": vc.n, "taps_per_direction": taps, "tile": args.tile,
-                   "chunk": args.chunk},
+                   "chunk": args.chunk, "waves": plan.waves or "default",
+                   "layout": args.layout},
         "phases_ms": {"pregrid_bin": round(float(t_bin), 4), "grid": round(float(t_grid), 4),
                       "degrid": round(float(t_degrid), 4)},
         "roofline": roof, "cpu_baseline": cpu, **extra,

@@ -309,6 +309,12 @@ rcg_status rcg_tiled_plan_create_ex(const rcg_geom *geoms, int n_inst,
                                     int tile, int chunk, int max_support,
                                     int layout, rcg_tiled_plan **plan);
 rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *plan);
+/* Wavefronts per workgroup of a pairs-layout plan's FP32 gridder, 1..4 (0:
+ * the default, 3 at tile 32, 4 otherwise).  Each holds a private FP32 copy
+ * of the tile in LDS, so the choice trades copies per workgroup against
+ * workgroups per CU; the autotuner (rc_amd/autotune.py) searches it with
+ * tile and chunk.  Not to be called while the plan's work is in flight. */
+rcg_status rcg_tiled_set_grid_waves(rcg_tiled_plan *plan, int waves);
 rcg_status rcg_tiled_bin(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                          void *stream);
 rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,

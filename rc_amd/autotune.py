@@ -1,19 +1,22 @@
-"""Autotuner of the tiled FP32 path: tile size and pairs per work item.
+"""Autotuner of the tiled FP32 path: tile size, pairs per work item and
+wavefronts per gridding workgroup.
 
 For a given workload (one geometry or a batch of instances, a GCF table and
-visibilities on the device) every candidate (tile, chunk) is built as a
-plan, binned, warmed up once and timed over `reps` bin + grid + degrid
+visibilities on the device) every candidate (tile, chunk, waves) is built as
+a plan, binned, warmed up once and timed over `reps` bin + grid + degrid
 passes with the kernel events the library records; the fastest candidate
-wins.  Candidates the plan rejects (too many tiles for the binning
-histograms, too many tiles per point) are reported and skipped.  Results
-are cached per workload signature (instances, points, grid shape, support
-histogram, tap count) in memory and, if `cache_path` is given, in a JSON
-file.
+wins.  Candidates the plan rejects (too many tiles per point for the
+support) are reported and skipped.  Results are cached per workload
+signature (instances, points, grid shape, support histogram, tap count) in
+memory and, if `cache_path` is given, in a JSON file.
 
-Tile size sets the LDS footprint per workgroup (4 wave-private FP32 copies
-of (T+1) x T cells for gridding) and with it the occupancy and how many
-tiles a window spans; chunk sets the pairs per work item, i.e. how many
-visibilities each wavefront walks between tile flushes.
+Tile size sets how many tiles a window spans and the LDS footprint of one
+wave-private FP32 copy of (T+1) x T cells; waves (1..4) sets how many such
+copies a gridding workgroup holds, i.e. the split of a CU's 160 KB of LDS
+between copies per workgroup and workgroups per CU; chunk sets the pairs
+per work item, i.e. how many visibilities each wavefront walks between tile
+flushes.  The plan for a candidate is binned once per tile and chunk and
+re-timed for each wave count (rcg_tiled_set_grid_waves).
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ from . import _lib
 from . import device as D
 
 DEFAULT_TILES = (16, 32, 64)
-DEFAULT_CHUNKS = (128, 256, 512, 1024)
+DEFAULT_CHUNKS = (256, 512, 1024, 2048, 4096)
+DEFAULT_WAVES = (1, 2, 3, 4)
 
 _memo: dict[str, dict] = {}
 
@@ -47,39 +51,51 @@ def signature(geoms, gcf: D.DeviceGCF) -> str:
     return h.hexdigest()
 
 
-def _time_candidate(geoms, gcf, vis, tile, chunk, max_support, reps):
-    plan = D.TiledPlan(geoms, tile=tile, chunk=chunk, max_support=max_support)
-    try:
-        grid = torch.empty(plan.grid_shape, dtype=torch.complex64, device=vis.device)
-        out = torch.empty_like(vis)
+def _time_plan(plan, gcf, vis, grid, out, reps):
+    plan.bin(gcf)
+    plan.grid(gcf, vis, grid)
+    plan.degrid(gcf, grid, out)
+    st = plan.stats()
+    if st["err"]:
+        raise _lib.RcgError(f"binning error word {st['err']}")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    plan.timing(True)
+    ev[0].record()
+    for _ in range(reps):
         plan.bin(gcf)
         plan.grid(gcf, vis, grid)
         plan.degrid(gcf, grid, out)
-        st = plan.stats()
-        if st["err"]:
-            raise _lib.RcgError(f"binning error word {st['err']}")
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        plan.timing(True)
-        ev[0].record()
-        for _ in range(reps):
-            plan.bin(gcf)
-            plan.grid(gcf, vis, grid)
-            plan.degrid(gcf, grid, out)
-        ev[1].record()
-        torch.cuda.synchronize()
-        tg, td = plan.kernel_times("grid"), plan.kernel_times("degrid")
-        return {"tile": tile, "chunk": chunk, "ms_step": ev[0].elapsed_time(ev[1]) / reps,
-                "ms_grid": float(np.mean(tg)), "ms_degrid": float(np.mean(td)),
-                "pairs": st["pairs"], "items": st["items"]}
+    ev[1].record()
+    torch.cuda.synchronize()
+    tg, td = plan.kernel_times("grid"), plan.kernel_times("degrid")
+    plan.timing(False)
+    return {"ms_step": ev[0].elapsed_time(ev[1]) / reps, "ms_grid": float(np.mean(tg)),
+            "ms_degrid": float(np.mean(td)), "pairs": st["pairs"], "items": st["items"]}
+
+
+def _time_candidates(geoms, gcf, vis, tile, chunk, waves, max_support, reps):
+    """One row per wave count for a (tile, chunk) plan."""
+    plan = D.TiledPlan(geoms, tile=tile, chunk=chunk, max_support=max_support)
+    rows = []
+    try:
+        grid = torch.empty(plan.grid_shape, dtype=torch.complex64, device=vis.device)
+        out = torch.empty_like(vis)
+        for w in waves:
+            plan.set_grid_waves(w)
+            rows.append({"tile": tile, "chunk": chunk, "waves": w,
+                         **_time_plan(plan, gcf, vis, grid, out, reps)})
     finally:
         plan.close()
+    return rows
 
 
 def autotune(geoms, gcf: D.DeviceGCF, vis: torch.Tensor, tiles=DEFAULT_TILES,
-             chunks=DEFAULT_CHUNKS, reps: int = 2, max_support: int | None = None,
-             cache_path: str | Path | None = None, log=None) -> dict:
-    """Returns {"tile", "chunk", "ms_step", "table": [...], "signature"}."""
-    sig = signature(geoms, gcf)
+             chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES, reps: int = 2,
+             max_support: int | None = None, cache_path: str | Path | None = None,
+             log=None) -> dict:
+    """Returns {"tile", "chunk", "waves", "ms_step", "table": [...],
+    "signature"}."""
+    sig = signature(geoms, gcf) + f"/{tuple(tiles)}/{tuple(chunks)}/{tuple(waves)}"
     if sig in _memo:
         return _memo[sig]
     if cache_path is not None and Path(cache_path).exists():
@@ -92,18 +108,22 @@ def autotune(geoms, gcf: D.DeviceGCF, vis: torch.Tensor, tiles=DEFAULT_TILES,
     for tile in tiles:
         for chunk in chunks:
             try:
-                r = _time_candidate(geoms, gcf, vis, tile, chunk, max_support, reps)
+                rows = _time_candidates(geoms, gcf, vis, tile, chunk, waves, max_support, reps)
             except _lib.RcgError as e:
-                r = {"tile": tile, "chunk": chunk, "ms_step": None, "skipped": str(e)}
-            table.append(r)
+                rows = [{"tile": tile, "chunk": chunk, "waves": None, "ms_step": None,
+                         "skipped": str(e)}]
+            table += rows
             if log:
-                log(f"autotune tile {tile} chunk {chunk}: {r.get('ms_step')}")
+                for r in rows:
+                    log(f"autotune tile {tile} chunk {chunk} waves {r.get('waves')}: "
+                        f"{r.get('ms_step')}")
     ok = [r for r in table if r.get("ms_step") is not None]
     if not ok:
         raise _lib.RcgError("autotune: no candidate could be planned")
     best = min(ok, key=lambda r: r["ms_step"])
-    res = {"tile": best["tile"], "chunk": best["chunk"], "ms_step": best["ms_step"],
-           "table": table, "signature": sig, "tune_s": time.perf_counter() - t0}
+    res = {"tile": best["tile"], "chunk": best["chunk"], "waves": best["waves"],
+           "ms_step": best["ms_step"], "table": table, "signature": sig,
+           "tune_s": time.perf_counter() - t0}
     _memo[sig] = res
     if cache_path is not None:
         p = Path(cache_path)

@@ -107,6 +107,7 @@ struct rcg_tiled_plan {
   long long n = 0;
   int max_support = 0;
   int max_pairs_pt = 0;
+  int grid_waves = 0;     // wavefronts (private tile copies) per gridding workgroup
   int grid_blocks = 0;    // persistent workgroups of the grid kernel
   int degrid_blocks = 0;  // ... of the degrid kernel
   int bin_blocks = 0;

@@ -403,17 +403,21 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 // 3-6% slower, 4 slabs 10-15%, 1 slab 13-17%).
 constexpr int GRP = 2;
 
-// Wavefronts per gridding workgroup.  Each holds a private copy of the tile
-// in LDS, so the copies' size sets how many workgroups share a CU's 160 KB:
-// at T = 32 three wavefronts (25 KB) let six workgroups (18 wavefronts) in,
-// against four of four (16); measured 2-3% faster on the 4096^2 w-projection
-// workload (one and two wavefronts: equal or slower).  At T = 64 three
-// copies would leave one 3-wavefront workgroup per CU.
+// Wavefronts per gridding workgroup (1..4, a plan's choice through
+// rcg_tiled_set_grid_waves; the autotuner searches it).  Each holds a
+// private copy of the tile in LDS, so the copies' size sets how many
+// workgroups share a CU's 160 KB: at T = 32 three wavefronts (25 KB) let six
+// workgroups (18 wavefronts) in, against four of four (16).  The default
+// below is the plan's choice until it is set: on the 4096^2 w-projection
+// workload 2, 3 and 4 wavefronts are within 1-2% of each other at T = 32,
+// one 3-5% slower; at T = 64 three copies leave one workgroup per CU.
 template <int T>
 constexpr int grid_waves()
 {
   return T == 32 ? 3 : 4;
 }
+// Wavefronts per gridding workgroup a plan may choose (rcg_tiled_set_grid_waves).
+constexpr int MAX_GRID_WAVES = 4;
 
 struct Pair {
   int h, rows, nslabs, point, whole, conj, slot;
@@ -474,13 +478,12 @@ __device__ __forceinline__ void load_group(float2 (&t)[GRP], int (&la)[GRP], con
 // zero to the dummy cell), and a wavefront's LDS operations complete in
 // program order; the copies are summed at the flush.
 
-template <int T>
-__global__ void __launch_bounds__(64 * grid_waves<T>())
+template <int T, int GW>
+__global__ void __launch_bounds__(64 * GW)
 k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
              const float2 *__restrict__ taps, long long n_taps,
              float2 *__restrict__ grid)
 {
-  constexpr int GW = grid_waves<T>();
   constexpr int LS = T + 1;
   constexpr int CP = T * LS + 2;  // one copy (+ dummy cell)
   constexpr int DUMMY = T * LS;
@@ -1010,15 +1013,45 @@ hipError_t persistent_blocks(const void *kernel, int block, int *nb)
   return hipSuccess;
 }
 
+// The gridding kernel of tile T with gw wavefronts per workgroup.
 template <int T>
-hipError_t tile_kernels(int *grid_blocks, int *degrid_blocks)
+const void *grid_kernel(int gw)
 {
-  hipError_t e = persistent_blocks(reinterpret_cast<const void *>(&k_grid_tiles<T>),
-                                   64 * grid_waves<T>(), grid_blocks);
-  if (e == hipSuccess)
+  switch (gw) {
+    case 1: return reinterpret_cast<const void *>(&k_grid_tiles<T, 1>);
+    case 2: return reinterpret_cast<const void *>(&k_grid_tiles<T, 2>);
+    case 3: return reinterpret_cast<const void *>(&k_grid_tiles<T, 3>);
+    default: return reinterpret_cast<const void *>(&k_grid_tiles<T, 4>);
+  }
+}
+
+template <int T>
+hipError_t tile_kernels(int gw, int *grid_blocks, int *degrid_blocks)
+{
+  hipError_t e = persistent_blocks(grid_kernel<T>(gw), 64 * gw, grid_blocks);
+  if (e == hipSuccess && degrid_blocks)
     e = persistent_blocks(reinterpret_cast<const void *>(&k_degrid_tiles<T>), 256,
                           degrid_blocks);
   return e;
+}
+
+template <int T, int GW>
+void launch_grid_tiles(int nb, const TiledDev &d, const float2 *v, const float2 *t,
+                       long long nt, float2 *g, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_grid_tiles<T, GW>), dim3(nb), dim3(64 * GW), 0, st, d, v, t, nt, g);
+}
+
+template <int T>
+void launch_grid_tiles(int gw, int nb, const TiledDev &d, const float2 *v, const float2 *t,
+                       long long nt, float2 *g, hipStream_t st)
+{
+  switch (gw) {
+    case 1: launch_grid_tiles<T, 1>(nb, d, v, t, nt, g, st); break;
+    case 2: launch_grid_tiles<T, 2>(nb, d, v, t, nt, g, st); break;
+    case 3: launch_grid_tiles<T, 3>(nb, d, v, t, nt, g, st); break;
+    default: launch_grid_tiles<T, 4>(nb, d, v, t, nt, g, st); break;
+  }
 }
 
 rcg_status check_tap_table(const rcg_gcf *gcf, const char *who)
@@ -1174,9 +1207,18 @@ extern "C" rcg_status rcg_tiled_plan_create_ex(const rcg_geom *geoms, int n_inst
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e == hipSuccess && !halo) {
     switch (tile) {
-      case 16: e = tile_kernels<16>(&pl->grid_blocks, &pl->degrid_blocks); break;
-      case 32: e = tile_kernels<32>(&pl->grid_blocks, &pl->degrid_blocks); break;
-      default: e = tile_kernels<64>(&pl->grid_blocks, &pl->degrid_blocks); break;
+      case 16:
+        pl->grid_waves = grid_waves<16>();
+        e = tile_kernels<16>(pl->grid_waves, &pl->grid_blocks, &pl->degrid_blocks);
+        break;
+      case 32:
+        pl->grid_waves = grid_waves<32>();
+        e = tile_kernels<32>(pl->grid_waves, &pl->grid_blocks, &pl->degrid_blocks);
+        break;
+      default:
+        pl->grid_waves = grid_waves<64>();
+        e = tile_kernels<64>(pl->grid_waves, &pl->grid_blocks, &pl->degrid_blocks);
+        break;
     }
   }
   if (e == hipSuccess && halo &&
@@ -1366,18 +1408,9 @@ rcg_status grid_tiled(rcg_tiled_plan *pl, const rcg_gcf *gcf, const rcg_c64 *vis
     if (s != RCG_OK) return s;
   } else {
     switch (d.T) {
-      case 16:
-        hipLaunchKernelGGL(k_grid_tiles<16>, nb, dim3(64 * grid_waves<16>()), 0, st, d, v, t, nt,
-                           g);
-        break;
-      case 32:
-        hipLaunchKernelGGL(k_grid_tiles<32>, nb, dim3(64 * grid_waves<32>()), 0, st, d, v, t, nt,
-                           g);
-        break;
-      default:
-        hipLaunchKernelGGL(k_grid_tiles<64>, nb, dim3(64 * grid_waves<64>()), 0, st, d, v, t, nt,
-                           g);
-        break;
+      case 16: launch_grid_tiles<16>(pl->grid_waves, pl->grid_blocks, d, v, t, nt, g, st); break;
+      case 32: launch_grid_tiles<32>(pl->grid_waves, pl->grid_blocks, d, v, t, nt, g, st); break;
+      default: launch_grid_tiles<64>(pl->grid_waves, pl->grid_blocks, d, v, t, nt, g, st); break;
     }
   }
   RCG_HIP_TRY(hipGetLastError());
@@ -1563,6 +1596,35 @@ rcg_status degrid_f64(rcg_tiled_plan *pl, const double2 *g, const double2 *t, lo
 }
 
 }  // namespace
+
+extern "C" rcg_status rcg_tiled_set_grid_waves(rcg_tiled_plan *pl, int waves)
+{
+  clear_error();
+  if (!pl) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_set_grid_waves: null plan");
+    return RCG_ERR_ARG;
+  }
+  if (pl->dev.layout != LAYOUT_PAIRS) {
+    set_error(RCG_ERR_UNSUPPORTED, "rcg_tiled_set_grid_waves: the halo layout's gridder runs "
+              "one wavefront per workgroup");
+    return RCG_ERR_UNSUPPORTED;
+  }
+  if (waves < 0 || waves > MAX_GRID_WAVES) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_set_grid_waves: waves %d not in [0, %d]", waves,
+              MAX_GRID_WAVES);
+    return RCG_ERR_ARG;
+  }
+  const int T = pl->dev.T;
+  if (waves == 0) waves = T == 16 ? grid_waves<16>() : T == 32 ? grid_waves<32>() : grid_waves<64>();
+  int nb = 0;
+  hipError_t e = T == 16 ? tile_kernels<16>(waves, &nb, nullptr)
+               : T == 32 ? tile_kernels<32>(waves, &nb, nullptr)
+                         : tile_kernels<64>(waves, &nb, nullptr);
+  RCG_HIP_TRY(e);
+  pl->grid_waves = waves;
+  pl->grid_blocks = nb;
+  return RCG_OK;
+}
 
 extern "C" rcg_status rcg_degrid_tiled_f64(rcg_tiled_plan *pl, const rcg_gcf *gcf,
                                            const rcg_c128 *grid, rcg_c128 *vis, void *stream)

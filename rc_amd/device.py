@@ -150,7 +150,7 @@ class TiledPlan(_lib.Handle):
     LAYOUTS = {"pairs": 0, "halo": 1, "halo16": 2}
 
     def __init__(self, geom, tile: int = 32, chunk: int = 512,
-                 max_support: int | None = None, layout: str = "pairs"):
+                 max_support: int | None = None, layout: str = "pairs", waves: int = 0):
         geoms = list(geom) if isinstance(geom, (list, tuple)) else [geom]
         self.batch = isinstance(geom, (list, tuple))
         self.geoms = geoms
@@ -171,6 +171,16 @@ class TiledPlan(_lib.Handle):
                                                        C.byref(h)),
                    "rcg_tiled_plan_create")
         self._own(h)
+        self.waves = 0
+        if waves:
+            self.set_grid_waves(waves)
+
+    def set_grid_waves(self, waves: int) -> None:
+        """Wavefronts (private tile copies) per gridding workgroup, 1..4;
+        0 restores the default (rcg_tiled_set_grid_waves)."""
+        _lib.check(_lib.lib().rcg_tiled_set_grid_waves(self._h, int(waves)),
+                   "rcg_tiled_set_grid_waves")
+        self.waves = int(waves)
 
     @property
     def grid_shape(self) -> tuple:

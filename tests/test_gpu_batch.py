@@ -156,10 +156,14 @@ def test_autotune_picks_a_timed_candidate(hip_device):
     from rc_amd import autotune
     vcs, gcf, N = small_batch(seed=12, n_inst=3, empty=False)
     geoms, dg, vis = upload(vcs, gcf, N)
-    res = autotune.autotune(geoms, dg, vis, tiles=(16, 32, 64), chunks=(64, 256), reps=1)
+    kw = dict(tiles=(16, 32, 64), chunks=(64, 256), waves=(1, 3), reps=1)
+    res = autotune.autotune(geoms, dg, vis, **kw)
     timed = [r for r in res["table"] if r["ms_step"] is not None]
-    assert len(res["table"]) == 6 and timed
-    assert (res["tile"], res["chunk"]) in [(r["tile"], r["chunk"]) for r in timed]
+    assert len(res["table"]) == 12 and timed
+    assert {(r["tile"], r["chunk"], r["waves"]) for r in timed} <= \
+        {(t, c, w) for t in (16, 32, 64) for c in (64, 256) for w in (1, 3)}
+    assert (res["tile"], res["chunk"], res["waves"]) in \
+        [(r["tile"], r["chunk"], r["waves"]) for r in timed]
     assert res["ms_step"] == min(r["ms_step"] for r in timed)
-    # memoised per workload signature
-    assert autotune.autotune(geoms, dg, vis) is res
+    # memoised per workload signature and sweep
+    assert autotune.autotune(geoms, dg, vis, **kw) is res

@@ -25,10 +25,10 @@ def c64_round(vc, gcf):
     return vc32, gcf.astype(np.complex64).astype(np.complex128)
 
 
-def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs"):
+def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs", waves=0):
     geom = D.DeviceGeom.from_chunks(vc, N)
     dg = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
-    plan = D.TiledPlan(geom, tile=tile, chunk=chunk, layout=layout)
+    plan = D.TiledPlan(geom, tile=tile, chunk=chunk, layout=layout, waves=waves)
     plan.bin(dg)
     vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).cuda()
     g = plan.grid(dg, vis)
@@ -65,6 +65,36 @@ def test_tiled_random_vs_oracle(hip_device, layout, tile, chunk, seed, half, cs)
     assert st["taps"] == taps
     assert rel_l2(g, ref) <= TOL
     assert rel_l2(v, vref) <= TOL
+
+
+@pytest.mark.parametrize("tile", [16, 32, 64])
+@pytest.mark.parametrize("waves", [1, 2, 3, 4])
+def test_tiled_grid_waves_vs_oracle(hip_device, tile, waves):
+    """Every choice of wavefronts (private tile copies) per gridding
+    workgroup, on the truncated-window case with chunks of 7 pairs (items
+    split over the wavefronts of a workgroup)."""
+    vc, gcf, _ = synth.random_small(4, grid_size=160, chunks=30, ts_ch=20, half=True,
+                                    chunk_supp="random", supports=(6, 17, 40))
+    N = 160
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N, nthreads=4)
+    g, v, st = run_tiled(vc, gcf, N, tile, 7, waves=waves)
+    assert st["err"] == 0 and st["taps"] == taps
+    assert rel_l2(g, ref) <= TOL
+
+
+def test_tiled_grid_waves_rejected(hip_device):
+    vc, gcf, N = synth.random_small(3)
+    geom = D.DeviceGeom.from_chunks(vc, N)
+    plan = D.TiledPlan(geom, tile=32, chunk=64)
+    from rc_amd import _lib
+    with pytest.raises(_lib.RcgError, match="waves"):
+        plan.set_grid_waves(5)
+    plan.close()
+    halo = D.TiledPlan(geom, tile=32, chunk=64, layout="halo")
+    with pytest.raises(_lib.RcgError, match="halo"):
+        halo.set_grid_waves(2)
+    halo.close()
 
 
 @pytest.mark.parametrize("layout,tile", [("pairs", 64), ("halo", 16), ("halo", 32)])
