@@ -108,11 +108,19 @@ def nv_sort_order(vis: np.ndarray) -> np.ndarray:
     return np.lexsort((sub, blk))
 
 
+def nv_split(xs: float):
+    """grid_gpu.cu:44-49: m = floor(xs), s = floor(8 (xs - m)), packed as
+    8 m + s; s = 8 (xs a hair below an integer, where xs - m rounds to 1.0)
+    carries into pixel m + 1, offset 0, as the packed value does."""
+    m = int(np.floor(xs))
+    s = int(np.floor(OVER * (xs - m)))
+    return (m + 1, s - OVER) if s >= OVER else (m, s)
+
+
 def nv_pixel(x: float, y: float, scale: float):
-    """grid_gpu.cu:37-51: (main_x, main_y, over) by floor, over = 8 sub_y + sub_x."""
-    xs, ys = x * scale, y * scale
-    mx, my = int(np.floor(xs)), int(np.floor(ys))
-    sx, sy = int(np.floor(OVER * (xs - mx))), int(np.floor(OVER * (ys - my)))
+    """grid_gpu.cu:37-51: (main_x, main_y, over), over = 8 sub_y + sub_x."""
+    mx, sx = nv_split(x * scale)
+    my, sy = nv_split(y * scale)
     return mx, my, OVER * sy + sx
 
 

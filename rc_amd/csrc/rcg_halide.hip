@@ -261,7 +261,7 @@ struct PlanHold {
 // per workgroup in LDS, tile sums added to the grid.  Taken for a square
 // grid window (extent x = extent y) with interleaved complex doubles (re/im
 // stride 1, x stride 2, even y stride) and a GCF of the same kind, where a
-// plan fits (at most 16384 tiles of 32 or 64, 2^31 pairs).  Returns 1 when
+// plan fits (at most MAX_PLAN_TILES tiles of 32 or 64, 2^31 pairs).  Returns 1 when
 // the points were gridded, 0 when the caller should run k_h_scatter, -1 on
 // a failure after work was queued (error set).
 int scatter_tiled(double scale, int grid_size, int margin, const buffer_t *vis,
@@ -291,14 +291,16 @@ int scatter_tiled(double scale, int grid_size, int margin, const buffer_t *vis,
   PlanHold h;
   for (int tile : {32, 64}) {
     const long long nt = (X + tile - 1) / tile;
-    if (nt * nt > 16384) continue;
+    if (nt * nt > MAX_PLAN_TILES) continue;
     if (rcg_tiled_plan_create(&g, tile, 1024, S, &h.p) == RCG_OK) break;
     h.p = nullptr;
   }
   clear_error();
+  (void)hipGetLastError();  // a failed plan allocation leaves its error behind
   if (!h.p) return 0;
   if (hipMalloc(reinterpret_cast<void **>(&h.v2), (size_t)n * sizeof(double2)) != hipSuccess) {
     h.v2 = nullptr;
+    (void)hipGetLastError();
     return 0;
   }
   hipLaunchKernelGGL(k_h_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d[0].p,
@@ -463,12 +465,22 @@ __device__ __forceinline__ bool nv_inside(int mx, int my, int img)
   return mx >= 7 && mx + 8 < img && my >= 7 && my + 8 < img;
 }
 
-// Sub-pixel offset floor(8 (xs - floor xs)), at most 7: for xs a hair below
-// an integer the difference rounds to 1.0, which would index the next row
-// of layers (and, in the last row, past the table).
-__device__ __forceinline__ int nv_sub(double xs, double fx)
+// Pixel m and sub-pixel offset s of a scaled coordinate xs:
+// m = floor(xs), s = floor(8 (xs - m)), as the reference packs them into
+// 8 m + s (MS6/kernel/nvidia/gridder/grid_gpu.cu:37-51).  For xs a hair
+// below an integer the difference rounds to 1.0 and s to 8, which the
+// packing carries into pixel m + 1 with s = 0 (e.g. xs = -1e-300: pixel 0,
+// offset 0); the same carry here.  (For other negative xs the reference's
+// unpacking by C division, grid_gpu.cu:117-125, would index before the
+// table; these stay at the floor pixel.)
+__device__ __forceinline__ void nv_split(double xs, double fx, int &m, int &s)
 {
-  return min((int)floor(H_OVER * (xs - fx)), H_OVER - 1);
+  m = (int)fx;
+  s = (int)floor(H_OVER * (xs - fx));
+  if (s >= H_OVER) {
+    m += 1;
+    s -= H_OVER;
+  }
 }
 
 // Records of the tiled FP64 gridder for nvGridder: per point whose window
@@ -492,10 +504,12 @@ k_nv_records(const double *__restrict__ pos, int ps, const double *__restrict__ 
   rec.supp = NV_GCF;
   rec.flags = 0;
   if (fx >= -8.0 && fx <= img + 6.0 && fy >= -8.0 && fy <= img + 6.0) {
-    const int mx = (int)fx, my = (int)fy;
+    int mx, my, sx, sy;
+    nv_split(xs, fx, mx, sx);
+    nv_split(ys, fy, my, sy);
     if (nv_inside(mx, my, img)) {
       rec.cell = (my - 7) * img + (mx - 7);
-      rec.tap0 = (H_OVER * nv_sub(ys, fy) + nv_sub(xs, fx)) * NV_TAPS;
+      rec.tap0 = (H_OVER * sy + sx) * NV_TAPS;
     }
   }
   pg[t] = rec;
@@ -518,9 +532,10 @@ k_nv_scatter(const double *__restrict__ pos, int ps, const double *__restrict__ 
   const double fx = floor(xs), fy = floor(ys);
   // no tap of a point outside these bounds lands in the grid (also NaN)
   if (!(fx >= -8.0 && fx <= img + 6.0 && fy >= -8.0 && fy <= img + 6.0)) return;
-  const int mx = (int)fx, my = (int)fy;
+  int mx, my, sx, sy;
+  nv_split(xs, fx, mx, sx);
+  nv_split(ys, fy, my, sy);
   if (skip_inside && nv_inside(mx, my, img)) return;
-  const int sx = nv_sub(xs, fx), sy = nv_sub(ys, fy);
   const double2 *g = gcf + (H_OVER * sy + sx) * NV_TAPS;
   const double vr = val[t * vs], vi = val[t * vs + 1];
   for (int k = lane; k < NV_TAPS; k += 64) {
@@ -633,11 +648,12 @@ int nv_grid_host(const char *who, double scale, double *out, const double *pos, 
       g.grid_pitch = img;
       for (int tile : {32, 64}) {
         const long long nt = (img + tile - 1) / tile;
-        if (nt * nt > 16384) continue;
+        if (nt * nt > MAX_PLAN_TILES) continue;
         if (rcg_tiled_plan_create(&g, tile, 1024, NV_GCF, &plan.p) == RCG_OK) break;
         plan.p = nullptr;
       }
       clear_error();
+      (void)hipGetLastError();  // a failed plan allocation leaves its error behind
       if (plan.p &&
           hipMalloc(reinterpret_cast<void **>(&plan.v2), (size_t)n * sizeof(double2)) ==
               hipSuccess) {
@@ -652,6 +668,7 @@ int nv_grid_host(const char *who, double scale, double *out, const double *pos, 
         tiled = 1;
       } else {
         plan.v2 = nullptr;
+        (void)hipGetLastError();  // the failed optional allocation: scatter instead
       }
     }
     if (e == hipSuccess && n > 0) {

@@ -161,6 +161,24 @@ def test_nv_gridder_edges_and_errors(hip_device):
                          HB.uvg_buffer(small)) == -1
 
 
+def test_nv_gridder_subpixel_carry(hip_device):
+    """x = -1e-300 (and y likewise): floor -1 with offset 8, carried to
+    pixel 0, offset 0 as the reference's packed coordinates do; both the
+    tiled route (a point inside) and the scatter route (points at the
+    edge)."""
+    N = 64
+    gcf = np.ascontiguousarray(np.random.default_rng(4).standard_normal((64, 16, 16, 2)))
+    vis = np.ascontiguousarray(np.array([[-1e-300, 20.0, 0, 1.0, 0.5],
+                                         [30.0, -1e-300, 0, 2.0, -1.0],
+                                         [40.0 - 2.0 ** -48, 33.0, 0, 1.0, 1.0]]))
+    exp = H.nv_grid(vis, gcf, 1.0, N)
+    assert H.nv_pixel(-1e-300, 20.0, 1.0) == (0, 20, 0)
+    uvg = np.zeros((N, N, 2))
+    assert HB.nv_gridder(1.0, N, HB.vis_buffer(vis), HB.nv_gcf_buffer(gcf),
+                         HB.uvg_buffer(uvg)) == 0
+    assert close(uvg, exp)
+
+
 def test_bin_sort_and_raw_gridders(hip_device):
     """bin_sort3 / bin_sort5 and gridGPUc / gridGPUs (grid_gpu.cuh:27-43)."""
     vis, gcf = nv_case(2000, 128, 1.0, 4)

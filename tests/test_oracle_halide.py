@@ -130,6 +130,24 @@ def test_nv_delta_tap_lands_on_its_pixel():
     assert H.nv_pixel(41.0, 61.5, 0.5) == (20, 30, 8 * 6 + 4)
 
 
+def test_nv_subpixel_carry():
+    """x a hair below an integer: 8 (x - floor x) rounds to 8, which the
+    reference's packing 8 main + sub (grid_gpu.cu:44-49) carries into the
+    next pixel with offset 0: x = -1e-300 is pixel 0, offset 0 (not pixel
+    -1 with offset 7 or 8)."""
+    x = -1e-300
+    assert np.floor(x) == -1.0 and np.floor(8 * (x - np.floor(x))) == 8.0
+    assert H.nv_split(x) == (0, 0)
+    assert H.nv_pixel(x, 5.25, 1.0) == (0, 5, 8 * 2 + 0)
+    assert H.nv_pixel(7.0 - 2.0 ** -50, 3.0, 1.0) == (6, 3, 7)  # an ordinary offset 7
+    # the point lands where its packed value says: taps of layer 8 sy + 0
+    N = 24
+    vis = np.array([[x, 10.0, 0.0, 1.0, 0.0]])
+    g = nv_delta(8 * 0 + 0, 7, 7)
+    out = H.nv_grid(vis, g, 1.0, N)
+    assert np.argwhere(out[..., 0] != 0).tolist() == [[10, 0]]
+
+
 def test_nv_taps_outside_the_grid_are_dropped():
     N = 32
     vis = np.array([[2.1, 2.1, 0.0, 1.0, 0.0]])
