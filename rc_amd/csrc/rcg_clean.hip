@@ -10,12 +10,16 @@
 // (The reference's own findPeak reads past the array and returns |x|, so its
 // results are not reproduced; see the oracle.)
 //
-// GPU structure: every iteration is one launch that subtracts the previous
-// peak's shifted PSF and forms per-block maxima of the updated residual
-// (wavefront reductions on (|value|, -index)), and one single-block launch
-// that reduces the block maxima, applies the threshold and updates the
-// model.  Peak, PSF peak, a stop flag and the iteration count live in device
-// memory, so no host synchronisation is needed between iterations.
+// GPU structure: one launch per iteration.  Every block of pass k first
+// reduces the block maxima that pass k - 1 left (512 entries of (|value|,
+// index, signed value): each block finds the same peak), applies the
+// threshold (block 0 updates the model and the state), subtracts the
+// peak's shifted PSF over its rows and leaves its own maximum of the
+// result (wavefront reductions on (|value|, -index)) in the other of two
+// buffers.  The kernel boundary orders the passes, so no block waits for
+// another and no device-wide fence is needed.  Peak, PSF peak, a stop flag
+// and the iteration count live in device memory, so no host
+// synchronisation is needed between iterations.
 #include <cfloat>
 
 #include "rcg_internal.h"
@@ -38,6 +42,14 @@ struct CleanState {
 struct Best {
   double a;        // |value|
   long long idx;   // row-major index row*siz + col
+  double v;        // the signed value
+};
+
+// Block maxima of one pass (CB entries).
+struct Partials {
+  double a[CB];
+  long long idx[CB];
+  double v[CB];
 };
 
 __device__ __forceinline__ Best better(Best x, Best y)
@@ -52,6 +64,7 @@ __device__ __forceinline__ Best wave_best(Best b)
     Best o;
     o.a = __shfl_xor(b.a, off);
     o.idx = __shfl_xor(b.idx, off);
+    o.v = __shfl_xor(b.v, off);
     b = better(b, o);
   }
   return b;
@@ -71,84 +84,98 @@ __device__ __forceinline__ Best block_best(Best b, Best *sh)
   return sh[0];
 }
 
-// Optional subtraction of the current peak, then partial maxima.
+// The maximum of a pass's block maxima (every thread gets it).
+__device__ __forceinline__ Best reduce_partials(const Partials *__restrict__ p, Best *sh)
+{
+  Best b{-1.0, 0x7fffffffffffffffLL, 0.0};
+  for (int k = threadIdx.x; k < CB; k += CT) b = better(b, Best{p->a[k], p->idx[k], p->v[k]});
+  return block_best(b, sh);
+}
+
+// The PSF peak from the PSF's block maxima (one block).
+__global__ void __launch_bounds__(CT)
+k_clean_psf_peak(CleanState *__restrict__ st, int siz, int pitch,
+                 const Partials *__restrict__ pin)
+{
+  __shared__ Best sh[CT / 64];
+  const Best b = reduce_partials(pin, sh);
+  if (threadIdx.x != 0) return;
+  st->ppos = (b.idx / siz) * pitch + b.idx % siz;
+  st->pval = b.v;
+  st->done = 0;
+  st->iters = 0;
+}
+
+// Pass `it` over the image.  it >= 1: reduce pass it - 1's block maxima
+// (pin) to the peak of iteration it - 1; stop below the threshold; else
+// block 0 adds gain * peak / psf peak to the model, and every block
+// subtracts that times the PSF shifted onto the peak.  find: leave this
+// block's maximum of the (updated) image in pout.  it = 0: find only.
+// The image may be the PSF itself (it = 0, no subtraction).
 template <class T>
 __global__ void __launch_bounds__(CT)
-k_sub_argmax(T *__restrict__ img, const T *__restrict__ psf, int siz, int pitch,
-             CleanState *__restrict__ st, double gain, int subtract, int find,
-             double *__restrict__ pa, long long *__restrict__ pi)
+k_clean_pass(T *__restrict__ img, const T *__restrict__ psf, T *__restrict__ model, int siz,
+             int pitch, CleanState *__restrict__ st, double gain, double threshold, int it,
+             int find, const Partials *__restrict__ pin, Partials *__restrict__ pout)
 {
 #pragma clang fp contract(off)
   __shared__ Best sh[CT / 64];
-  const bool act = !st->done;
   long long dr = 0, dc = 0;
   T a = 0;
-  if (subtract && act) {
-    const long long rr = st->pos / pitch, rc = st->pos % pitch;
+  const bool subtract = it > 0;
+  if (subtract) {
+    if (st->done) return;
+    const Best pk = reduce_partials(pin, sh);
+    const long long r = pk.idx / siz, c = pk.idx % siz;
+    const long long pos = r * pitch + c;
+    const double pval = st->pval;
+    if (fabs(pk.v) < threshold || pval == 0.0) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pos = pos;
+        st->val = pk.v;
+        st->done = 1;
+      }
+      return;
+    }
+    const double g = gain * pk.v / pval;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->pos = pos;
+      st->val = pk.v;
+      st->iters = it;
+      model[pos] += (T)g;
+    }
     const long long pr = st->ppos / pitch, pc = st->ppos % pitch;
-    dr = pr - rr;
-    dc = pc - rc;
-    a = (T)(gain * st->val / st->pval);
+    dr = pr - r;
+    dc = pc - c;
+    a = (T)g;
   }
-  Best b{-1.0, 0x7fffffffffffffffLL};
+  Best b{-1.0, 0x7fffffffffffffffLL, 0.0};
   for (int r = blockIdx.x; r < siz; r += gridDim.x) {
     T *row = img + (long long)r * pitch;
     const long long sr = r + dr;
     const bool in_r = sr >= 0 && sr < siz;
+    // unrolled so that a thread's loads of several columns are in flight
+    // together (img and psf do not alias)
+#pragma unroll 4
     for (int c = threadIdx.x; c < siz; c += CT) {
       T x = row[c];
-      if (subtract && act && in_r) {
+      if (subtract && in_r) {
         const long long sc = c + dc;
         if (sc >= 0 && sc < siz) {
           x -= a * psf[sr * pitch + sc];
           row[c] = x;
         }
       }
-      if (find) b = better(b, Best{(double)fabs(x), (long long)r * siz + c});
+      if (find) b = better(b, Best{(double)fabs(x), (long long)r * siz + c, (double)x});
     }
   }
-  if (find) {
-    b = block_best(b, sh);
-    if (threadIdx.x == 0) {
-      pa[blockIdx.x] = b.a;
-      pi[blockIdx.x] = b.idx;
-    }
-  }
-}
-
-// mode 0: PSF peak; mode 1: residual peak of iteration `it`
-template <class T>
-__global__ void __launch_bounds__(CT)
-k_clean_final(const T *__restrict__ img, T *__restrict__ model, int siz, int pitch,
-              CleanState *__restrict__ st, const double *__restrict__ pa,
-              const long long *__restrict__ pi, int nb, int mode, int it, double gain,
-              double threshold)
-{
-#pragma clang fp contract(off)
-  __shared__ Best sh[CT / 64];
-  if (mode == 1 && st->done) return;
-  Best b{-1.0, 0x7fffffffffffffffLL};
-  for (int k = threadIdx.x; k < nb; k += CT) b = better(b, Best{pa[k], pi[k]});
+  if (!find) return;
   b = block_best(b, sh);
-  if (threadIdx.x != 0) return;
-  const long long r = b.idx / siz, c = b.idx % siz;
-  const long long pos = r * pitch + c;
-  const double v = (double)img[pos];
-  if (mode == 0) {
-    st->ppos = pos;
-    st->pval = v;
-    st->done = 0;
-    st->iters = 0;
-    return;
+  if (threadIdx.x == 0) {
+    pout->a[blockIdx.x] = b.a;
+    pout->idx[blockIdx.x] = b.idx;
+    pout->v[blockIdx.x] = b.v;
   }
-  st->pos = pos;
-  st->val = v;
-  if (fabs(v) < threshold || st->pval == 0.0) {
-    st->done = 1;
-    return;
-  }
-  model[pos] += (T)(gain * v / st->pval);
-  st->iters = it + 1;
 }
 
 template <class T>
@@ -164,33 +191,23 @@ rcg_status hogbom(T *model, T *res, const T *psf, int siz, int pitch, unsigned n
                   double gain, double threshold, void *scratch, hipStream_t st)
 {
   CleanState *cs = static_cast<CleanState *>(scratch);
-  double *pa = reinterpret_cast<double *>(cs + 1);
-  long long *pi = reinterpret_cast<long long *>(pa + CB);
+  Partials *pp = reinterpret_cast<Partials *>(cs + 1);  // two buffers, passes alternate
   const long long n = (long long)siz * pitch;
   hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(256), 0, st, model, n);
   // PSF peak
-  hipLaunchKernelGGL(k_sub_argmax<T>, dim3(CB), dim3(CT), 0, st, const_cast<T *>(psf), psf,
-                     siz, pitch, cs, gain, 0, 1, pa, pi);
-  hipLaunchKernelGGL(k_clean_final<T>, dim3(1), dim3(CT), 0, st, psf, model, siz, pitch, cs,
-                     pa, pi, CB, 0, 0, gain, threshold);
+  hipLaunchKernelGGL(k_clean_pass<T>, dim3(CB), dim3(CT), 0, st, const_cast<T *>(psf), psf,
+                     model, siz, pitch, cs, gain, threshold, 0, 1, pp + 1, pp);
+  hipLaunchKernelGGL(k_clean_psf_peak, dim3(1), dim3(CT), 0, st, cs, siz, pitch, pp);
   if (niters == 0) {
     RCG_HIP_TRY(hipGetLastError());
     return RCG_OK;
   }
-  // first residual peak
-  hipLaunchKernelGGL(k_sub_argmax<T>, dim3(CB), dim3(CT), 0, st, res, psf, siz, pitch, cs,
-                     gain, 0, 1, pa, pi);
-  hipLaunchKernelGGL(k_clean_final<T>, dim3(1), dim3(CT), 0, st, res, model, siz, pitch, cs,
-                     pa, pi, CB, 1, 0, gain, threshold);
-  for (unsigned it = 1; it < niters; it++) {
-    hipLaunchKernelGGL(k_sub_argmax<T>, dim3(CB), dim3(CT), 0, st, res, psf, siz, pitch, cs,
-                       gain, 1, 1, pa, pi);
-    hipLaunchKernelGGL(k_clean_final<T>, dim3(1), dim3(CT), 0, st, res, model, siz, pitch, cs,
-                       pa, pi, CB, 1, (int)it, gain, threshold);
-  }
-  // subtraction of the last accepted peak
-  hipLaunchKernelGGL(k_sub_argmax<T>, dim3(CB), dim3(CT), 0, st, res, psf, siz, pitch, cs,
-                     gain, 1, 0, pa, pi);
+  // pass 0: the maxima of the residual; pass it (1..niters): peak it - 1,
+  // its subtraction and, but for the last pass, the next maxima
+  for (unsigned it = 0; it <= niters; it++)
+    hipLaunchKernelGGL(k_clean_pass<T>, dim3(CB), dim3(CT), 0, st, res, psf, model, siz, pitch,
+                       cs, gain, threshold, (int)it, it < niters ? 1 : 0, pp + ((it + 1) & 1),
+                       pp + (it & 1));
   RCG_HIP_TRY(hipGetLastError());
   return RCG_OK;
 }
@@ -202,7 +219,7 @@ using namespace rcg;
 
 extern "C" size_t rcg_hogbom_scratch_bytes(void)
 {
-  return sizeof(CleanState) + CB * (sizeof(double) + sizeof(long long));
+  return sizeof(CleanState) + 2 * sizeof(Partials);
 }
 
 extern "C" rcg_status rcg_hogbom(void *model, void *res, const void *psf, int f64, int siz,
