@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--waves", type=int, default=0,
                     help="wavefronts per gridding workgroup, 1..4 (0: the plan's default)")
+    ap.add_argument("--degrid-waves", type=int, default=0,
+                    help="wavefronts per degridding workgroup, 2, 4 or 8 (0: the default 4)")
     ap.add_argument("--layout", choices=["pairs", "halo", "halo16"], default="pairs",
                     help="record layout of the tiled plan (see rcg_tiled_plan_create_ex)")
     ap.add_argument("--n-times", type=int, default=306)
@@ -154,9 +156,11 @@ def bench_batched(args) -> int:
     vis = torch.from_numpy(np.concatenate([v.vis.reshape(-1) for v in vcs])
                            .astype(np.complex64)).to(dev)
     tune = autotune.autotune(geoms, dg, vis, max_support=48, log=log)
-    tile, chunk, waves = tune["tile"], tune["chunk"], tune["waves"]
-    log(f"autotune: tile {tile} chunk {chunk} waves {waves} ({tune['tune_s']:.1f} s)")
-    plan = D.TiledPlan(geoms, tile=tile, chunk=chunk, max_support=48, waves=waves)
+    tile, chunk, waves, dwaves = tune["tile"], tune["chunk"], tune["waves"], tune["degrid_waves"]
+    log(f"autotune: tile {tile} chunk {chunk} waves {waves} degrid waves {dwaves} "
+        f"({tune['tune_s']:.1f} s)")
+    plan = D.TiledPlan(geoms, tile=tile, chunk=chunk, max_support=48, waves=waves,
+                       degrid_waves=dwaves)
     grid = torch.empty(plan.grid_shape, dtype=torch.complex64, device=dev)
     out = torch.empty(n, dtype=torch.complex64, device=dev)
     graph = plan.capture(dg, D.STAGE_BIN | D.STAGE_GRID | D.STAGE_DEGRID, vis=vis, grid=grid,
@@ -179,8 +183,8 @@ def bench_batched(args) -> int:
         return 1
     # one plan and one launch sequence per instance, same tile and chunk
     offs = np.cumsum([0] + [g.n for g in geoms])
-    plans = [D.TiledPlan(g, tile=tile, chunk=chunk, max_support=48, waves=waves)
-             for g in geoms]
+    plans = [D.TiledPlan(g, tile=tile, chunk=chunk, max_support=48, waves=waves,
+                         degrid_waves=dwaves) for g in geoms]
 
     def per_instance():
         for k, p in enumerate(plans):
@@ -230,7 +234,7 @@ def bench_batched(args) -> int:
                                "supports 8-48, lognormal counts (median 400k, sigma 1, "
                                "20k-5M)",
                    "points": n, "taps_per_direction": st["taps"], "tile": tile,
-                   "chunk": chunk, "waves": waves},
+                   "chunk": chunk, "waves": waves, "degrid_waves": dwaves},
         "per_instance_ms_per_step": round(ms_single, 4),
         "per_instance_graphs_ms_per_step": round(ms_single_graph, 4),
         "speedup_vs_per_instance": round(ms_single / ms_batch, 3),
@@ -238,6 +242,8 @@ def bench_batched(args) -> int:
         "autotune": [{k: r.get(k) for k in ("tile", "chunk", "waves", "ms_step", "ms_grid",
                                             "ms_degrid", "skipped")}
                      for r in tune["table"]],
+        "autotune_degrid_waves": [{k: r.get(k) for k in ("degrid_waves", "ms_degrid")}
+                                  for r in tune["degrid_table"]],
     }
     print(json.dumps(line), flush=True)
     graph.close()
@@ -934,7 +940,10 @@ def bench_tune(args) -> int:
     vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).cuda()
     tune = autotune.autotune(geom, dg, vis, max_support=64, reps=max(2, args.steps), log=log)
     line = {"workload": "wproj 4096x4096, 32 w-planes, supports 8-64", "points": vc.n,
-            "best": {k: tune[k] for k in ("tile", "chunk", "waves", "ms_step")},
+            "best": {k: tune[k] for k in ("tile", "chunk", "waves", "degrid_waves", "ms_step")},
+            "degrid_table": [{k: round(r[k], 4) if isinstance(r[k], float) else r[k]
+                              for k in ("degrid_waves", "ms_degrid")}
+                             for r in tune["degrid_table"]],
             "tune_s": round(tune["tune_s"], 1),
             "table": [{k: (round(r[k], 4) if isinstance(r.get(k), float) else r.get(k))
                        for k in ("tile", "chunk", "waves", "ms_step", "ms_grid", "ms_degrid",
@@ -983,7 +992,7 @@ def main() -> int:
     dg32 = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
     vis32 = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).to(dev)
     plan = D.TiledPlan(geom, tile=args.tile, chunk=args.chunk, max_support=64,
-                       layout=args.layout, waves=args.waves)
+                       layout=args.layout, waves=args.waves, degrid_waves=args.degrid_waves)
     grid32 = torch.empty((N, N), dtype=torch.complex64, device=dev)
     out32 = torch.empty(vc.n, dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream()
@@ -1107,7 +1116,7 @@ def main() -> int:
     fp32_tf = 8 * taps / sec / 1e12
     b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
     default_cfg = (args.tile == 32 and args.chunk == 1024 and args.n_times == 306 and
-                   args.layout == "pairs" and args.waves == 0)
+                   args.layout == "pairs" and args.waves == 0 and args.degrid_waves == 0)
     prof = profiled(kname) if default_cfg else {}
     tb = prof.get("traffic_bytes")
     hbm_gbs = tb / sec / 1e9 if tb else None
@@ -1158,6 +1167,7 @@ def main() -> int:
                                "supports 8-64, 8x oversampling",
                    "points": vc.n, "taps_per_direction": taps, "tile": args.tile,
                    "chunk": args.chunk, "waves": plan.waves or "default",
+                   "degrid_waves": plan.degrid_waves or "default",
                    "layout": args.layout},
         "phases_ms": {"pregrid_bin": round(float(t_bin), 4), "grid": round(float(t_grid), 4),
                       "degrid": round(float(t_degrid), 4)},

@@ -315,6 +315,10 @@ rcg_status rcg_tiled_plan_destroy(rcg_tiled_plan *plan);
  * workgroups per CU; the autotuner (rc_amd/autotune.py) searches it with
  * tile and chunk.  Not to be called while the plan's work is in flight. */
 rcg_status rcg_tiled_set_grid_waves(rcg_tiled_plan *plan, int waves);
+/* Wavefronts per workgroup of a pairs-layout plan's FP32 degridder: 2, 4 or 8
+ * (0: the default 4).  They share one staged copy of the tile.  Not to be
+ * called while the plan's work is in flight. */
+rcg_status rcg_tiled_set_degrid_waves(rcg_tiled_plan *plan, int waves);
 rcg_status rcg_tiled_bin(rcg_tiled_plan *plan, const rcg_gcf *gcf,
                          void *stream);
 rcg_status rcg_grid_tiled_f32(rcg_tiled_plan *plan, const rcg_gcf *gcf,

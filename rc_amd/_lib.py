@@ -86,6 +86,7 @@ SIGNATURES: dict[str, tuple] = {
                                          C.POINTER(vp)]),
     "rcg_graph_launch": (C.c_int, [vp, vp]),
     "rcg_graph_destroy": (C.c_int, [vp]),
+    "rcg_tiled_set_degrid_waves": (C.c_int, [vp, C.c_int]),
     "rcg_tiled_set_grid_waves": (C.c_int, [vp, C.c_int]),
     "rcg_grid_tiled_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),
     "rcg_grid_tiled_adjoint_f32": (C.c_int, [vp, C.POINTER(Gcf), vp, vp, vp]),

@@ -34,6 +34,7 @@ from . import device as D
 DEFAULT_TILES = (16, 32, 64)
 DEFAULT_CHUNKS = (256, 512, 1024, 2048, 4096)
 DEFAULT_WAVES = (1, 2, 3, 4)
+DEFAULT_DEGRID_WAVES = (2, 4, 8)
 
 _memo: dict[str, dict] = {}
 
@@ -89,13 +90,32 @@ def _time_candidates(geoms, gcf, vis, tile, chunk, waves, max_support, reps):
     return rows
 
 
+def _time_degrid_waves(geoms, gcf, vis, best, dwaves, max_support, reps):
+    """The chosen (tile, chunk, waves) plan re-timed per degrid wave count."""
+    plan = D.TiledPlan(geoms, tile=best["tile"], chunk=best["chunk"], max_support=max_support,
+                       waves=best["waves"])
+    rows = []
+    try:
+        grid = torch.empty(plan.grid_shape, dtype=torch.complex64, device=vis.device)
+        out = torch.empty_like(vis)
+        for dw in dwaves:
+            plan.set_degrid_waves(dw)
+            rows.append({"degrid_waves": dw, **_time_plan(plan, gcf, vis, grid, out, reps)})
+    finally:
+        plan.close()
+    return rows
+
+
 def autotune(geoms, gcf: D.DeviceGCF, vis: torch.Tensor, tiles=DEFAULT_TILES,
-             chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES, reps: int = 2,
-             max_support: int | None = None, cache_path: str | Path | None = None,
-             log=None) -> dict:
-    """Returns {"tile", "chunk", "waves", "ms_step", "table": [...],
-    "signature"}."""
-    sig = signature(geoms, gcf) + f"/{tuple(tiles)}/{tuple(chunks)}/{tuple(waves)}"
+             chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES, degrid_waves=DEFAULT_DEGRID_WAVES,
+             reps: int = 2, max_support: int | None = None,
+             cache_path: str | Path | None = None, log=None) -> dict:
+    """Returns {"tile", "chunk", "waves", "degrid_waves", "ms_step", "table":
+    [...], "degrid_table": [...], "signature"}.  The degridder's wave count
+    is searched after the others, on the chosen plan (the degridder's time
+    does not depend on the gridder's wave count)."""
+    sig = (signature(geoms, gcf)
+           + f"/{tuple(tiles)}/{tuple(chunks)}/{tuple(waves)}/{tuple(degrid_waves)}")
     if sig in _memo:
         return _memo[sig]
     if cache_path is not None and Path(cache_path).exists():
@@ -121,9 +141,15 @@ def autotune(geoms, gcf: D.DeviceGCF, vis: torch.Tensor, tiles=DEFAULT_TILES,
     if not ok:
         raise _lib.RcgError("autotune: no candidate could be planned")
     best = min(ok, key=lambda r: r["ms_step"])
+    dtable = _time_degrid_waves(geoms, gcf, vis, best, degrid_waves, max_support, reps) \
+        if degrid_waves else []
+    dbest = min(dtable, key=lambda r: r["ms_degrid"]) if dtable else {"degrid_waves": 0}
+    if log:
+        for r in dtable:
+            log(f"autotune degrid waves {r['degrid_waves']}: degrid {r['ms_degrid']:.4f} ms")
     res = {"tile": best["tile"], "chunk": best["chunk"], "waves": best["waves"],
-           "ms_step": best["ms_step"], "table": table, "signature": sig,
-           "tune_s": time.perf_counter() - t0}
+           "degrid_waves": dbest["degrid_waves"], "ms_step": best["ms_step"], "table": table,
+           "degrid_table": dtable, "signature": sig, "tune_s": time.perf_counter() - t0}
     _memo[sig] = res
     if cache_path is not None:
         p = Path(cache_path)

@@ -108,6 +108,7 @@ struct rcg_tiled_plan {
   int max_support = 0;
   int max_pairs_pt = 0;
   int grid_waves = 0;     // wavefronts (private tile copies) per gridding workgroup
+  int degrid_waves = 4;   // wavefronts per degridding workgroup (sharing the staged tile)
   int grid_blocks = 0;    // persistent workgroups of the grid kernel
   int degrid_blocks = 0;  // ... of the degrid kernel
   int bin_blocks = 0;

@@ -820,8 +820,8 @@ __global__ void k_comp_combine(float2 *__restrict__ grid, const float2 *__restri
 // lanes accumulate four partial products, which are summed across the
 // wavefront four pairs at a time.
 
-template <int T>
-__global__ void __launch_bounds__(256)
+template <int T, int DW>
+__global__ void __launch_bounds__(64 * DW)
 k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
                const float2 *__restrict__ taps, long long n_taps,
                float2 *__restrict__ vis)
@@ -829,10 +829,10 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
   constexpr int LS = T + 1;
   constexpr int DUMMY = T * LS;
   __shared__ float2 tile[T * LS + 2];
-  __shared__ float2 sums[4][SUMS * 64];
+  __shared__ float2 sums[DW][SUMS * 64];
   // output codes of the wavefront's pairs k mod 128: two batches of 64, the
   // current one and the one whose last sums may still be pending
-  __shared__ int codes[4][128];
+  __shared__ int codes[DW][128];
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -842,7 +842,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
   for_each_item(d, d.queue + 8, [&](const ItemRange &ir) {
     const int gsz = d.grid_size;
     const float2 *gi = grid + ir.goff;
-    for (int q = threadIdx.x; q < T * T; q += 256) {
+    for (int q = threadIdx.x; q < T * T; q += 64 * DW) {
       const int r = q / T, c = q - (q / T) * T;
       const int gu = ir.tu0 + r, gv = ir.tv0 + c;
       tile[r * LS + c] = (gu < gsz && gv < gsz) ? gi[(long long)gu * d.pitch + gv]
@@ -850,12 +850,12 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
     }
     if (threadIdx.x == 0) tile[DUMMY] = make_float2(0.f, 0.f);
     __syncthreads();
-    const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + 3) / 4 : 0;
+    const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + DW - 1) / DW : 0;
     if (npw > 0) {
       int4 rec, rec_n;
       float2 dm, dm_n;
-      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm, 4);
-      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n, 4);
+      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm, DW);
+      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n, DW);
       int k = 0, s0 = 0;
       my_codes[lane] = out_code(rec, npts);
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
@@ -887,7 +887,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
               // previous one, whose sums are all stored
               rec = rec_n;
               my_codes[(kn & 64) + lane] = out_code(rec, npts);
-              stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n, 4);
+              stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n, DW);
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  0);
@@ -1025,14 +1025,44 @@ const void *grid_kernel(int gw)
   }
 }
 
+// The degridding kernel of tile T with dw wavefronts per workgroup.
 template <int T>
-hipError_t tile_kernels(int gw, int *grid_blocks, int *degrid_blocks)
+const void *degrid_kernel(int dw)
 {
-  hipError_t e = persistent_blocks(grid_kernel<T>(gw), 64 * gw, grid_blocks);
+  switch (dw) {
+    case 2: return reinterpret_cast<const void *>(&k_degrid_tiles<T, 2>);
+    case 8: return reinterpret_cast<const void *>(&k_degrid_tiles<T, 8>);
+    default: return reinterpret_cast<const void *>(&k_degrid_tiles<T, 4>);
+  }
+}
+
+// Persistent workgroups of the grid kernel (gw wavefronts) and, when
+// degrid_blocks is given, of the degrid kernel (dw wavefronts).
+template <int T>
+hipError_t tile_kernels(int gw, int dw, int *grid_blocks, int *degrid_blocks)
+{
+  hipError_t e = grid_blocks ? persistent_blocks(grid_kernel<T>(gw), 64 * gw, grid_blocks)
+                             : hipSuccess;
   if (e == hipSuccess && degrid_blocks)
-    e = persistent_blocks(reinterpret_cast<const void *>(&k_degrid_tiles<T>), 256,
-                          degrid_blocks);
+    e = persistent_blocks(degrid_kernel<T>(dw), 64 * dw, degrid_blocks);
   return e;
+}
+
+template <int T>
+void launch_degrid_tiles(int dw, int nb, const TiledDev &d, const float2 *g, const float2 *t,
+                         long long nt, float2 *v, hipStream_t st)
+{
+  switch (dw) {
+    case 2:
+      hipLaunchKernelGGL((k_degrid_tiles<T, 2>), dim3(nb), dim3(128), 0, st, d, g, t, nt, v);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_degrid_tiles<T, 8>), dim3(nb), dim3(512), 0, st, d, g, t, nt, v);
+      break;
+    default:
+      hipLaunchKernelGGL((k_degrid_tiles<T, 4>), dim3(nb), dim3(256), 0, st, d, g, t, nt, v);
+      break;
+  }
 }
 
 template <int T, int GW>
@@ -1209,15 +1239,18 @@ extern "C" rcg_status rcg_tiled_plan_create_ex(const rcg_geom *geoms, int n_inst
     switch (tile) {
       case 16:
         pl->grid_waves = grid_waves<16>();
-        e = tile_kernels<16>(pl->grid_waves, &pl->grid_blocks, &pl->degrid_blocks);
+        e = tile_kernels<16>(pl->grid_waves, pl->degrid_waves, &pl->grid_blocks,
+                             &pl->degrid_blocks);
         break;
       case 32:
         pl->grid_waves = grid_waves<32>();
-        e = tile_kernels<32>(pl->grid_waves, &pl->grid_blocks, &pl->degrid_blocks);
+        e = tile_kernels<32>(pl->grid_waves, pl->degrid_waves, &pl->grid_blocks,
+                             &pl->degrid_blocks);
         break;
       default:
         pl->grid_waves = grid_waves<64>();
-        e = tile_kernels<64>(pl->grid_waves, &pl->grid_blocks, &pl->degrid_blocks);
+        e = tile_kernels<64>(pl->grid_waves, pl->degrid_waves, &pl->grid_blocks,
+                             &pl->degrid_blocks);
         break;
     }
   }
@@ -1489,9 +1522,9 @@ extern "C" rcg_status rcg_degrid_tiled_f32(rcg_tiled_plan *pl, const rcg_gcf *gc
     if (s != RCG_OK) return s;
   } else {
     switch (d.T) {
-      case 16: hipLaunchKernelGGL(k_degrid_tiles<16>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
-      case 32: hipLaunchKernelGGL(k_degrid_tiles<32>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
-      default: hipLaunchKernelGGL(k_degrid_tiles<64>, nb, dim3(256), 0, st, d, g, t, nt, v); break;
+      case 16: launch_degrid_tiles<16>(pl->degrid_waves, pl->degrid_blocks, d, g, t, nt, v, st); break;
+      case 32: launch_degrid_tiles<32>(pl->degrid_waves, pl->degrid_blocks, d, g, t, nt, v, st); break;
+      default: launch_degrid_tiles<64>(pl->degrid_waves, pl->degrid_blocks, d, g, t, nt, v, st); break;
     }
   }
   RCG_HIP_TRY(hipGetLastError());
@@ -1617,12 +1650,40 @@ extern "C" rcg_status rcg_tiled_set_grid_waves(rcg_tiled_plan *pl, int waves)
   const int T = pl->dev.T;
   if (waves == 0) waves = T == 16 ? grid_waves<16>() : T == 32 ? grid_waves<32>() : grid_waves<64>();
   int nb = 0;
-  hipError_t e = T == 16 ? tile_kernels<16>(waves, &nb, nullptr)
-               : T == 32 ? tile_kernels<32>(waves, &nb, nullptr)
-                         : tile_kernels<64>(waves, &nb, nullptr);
+  hipError_t e = T == 16 ? tile_kernels<16>(waves, 0, &nb, nullptr)
+               : T == 32 ? tile_kernels<32>(waves, 0, &nb, nullptr)
+                         : tile_kernels<64>(waves, 0, &nb, nullptr);
   RCG_HIP_TRY(e);
   pl->grid_waves = waves;
   pl->grid_blocks = nb;
+  return RCG_OK;
+}
+
+extern "C" rcg_status rcg_tiled_set_degrid_waves(rcg_tiled_plan *pl, int waves)
+{
+  clear_error();
+  if (!pl) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_set_degrid_waves: null plan");
+    return RCG_ERR_ARG;
+  }
+  if (pl->dev.layout != LAYOUT_PAIRS) {
+    set_error(RCG_ERR_UNSUPPORTED, "rcg_tiled_set_degrid_waves: the halo layout's degridder "
+              "runs one wavefront per workgroup");
+    return RCG_ERR_UNSUPPORTED;
+  }
+  if (waves == 0) waves = 4;
+  if (waves != 2 && waves != 4 && waves != 8) {
+    set_error(RCG_ERR_ARG, "rcg_tiled_set_degrid_waves: waves %d not 2, 4 or 8 (0: 4)", waves);
+    return RCG_ERR_ARG;
+  }
+  const int T = pl->dev.T;
+  int nb = 0;
+  hipError_t e = T == 16 ? tile_kernels<16>(0, waves, nullptr, &nb)
+               : T == 32 ? tile_kernels<32>(0, waves, nullptr, &nb)
+                         : tile_kernels<64>(0, waves, nullptr, &nb);
+  RCG_HIP_TRY(e);
+  pl->degrid_waves = waves;
+  pl->degrid_blocks = nb;
   return RCG_OK;
 }
 

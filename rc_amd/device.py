@@ -150,7 +150,8 @@ class TiledPlan(_lib.Handle):
     LAYOUTS = {"pairs": 0, "halo": 1, "halo16": 2}
 
     def __init__(self, geom, tile: int = 32, chunk: int = 512,
-                 max_support: int | None = None, layout: str = "pairs", waves: int = 0):
+                 max_support: int | None = None, layout: str = "pairs", waves: int = 0,
+                 degrid_waves: int = 0):
         geoms = list(geom) if isinstance(geom, (list, tuple)) else [geom]
         self.batch = isinstance(geom, (list, tuple))
         self.geoms = geoms
@@ -172,8 +173,18 @@ class TiledPlan(_lib.Handle):
                    "rcg_tiled_plan_create")
         self._own(h)
         self.waves = 0
+        self.degrid_waves = 0
         if waves:
             self.set_grid_waves(waves)
+        if degrid_waves:
+            self.set_degrid_waves(degrid_waves)
+
+    def set_degrid_waves(self, waves: int) -> None:
+        """Wavefronts per degridding workgroup, 2, 4 or 8; 0 restores the
+        default 4 (rcg_tiled_set_degrid_waves)."""
+        _lib.check(_lib.lib().rcg_tiled_set_degrid_waves(self._h, int(waves)),
+                   "rcg_tiled_set_degrid_waves")
+        self.degrid_waves = int(waves)
 
     def set_grid_waves(self, waves: int) -> None:
         """Wavefronts (private tile copies) per gridding workgroup, 1..4;

@@ -165,5 +165,6 @@ def test_autotune_picks_a_timed_candidate(hip_device):
     assert (res["tile"], res["chunk"], res["waves"]) in \
         [(r["tile"], r["chunk"], r["waves"]) for r in timed]
     assert res["ms_step"] == min(r["ms_step"] for r in timed)
+    assert res["degrid_waves"] in (2, 4, 8) and len(res["degrid_table"]) == 3
     # memoised per workload signature and sweep
     assert autotune.autotune(geoms, dg, vis, **kw) is res

@@ -25,10 +25,11 @@ def c64_round(vc, gcf):
     return vc32, gcf.astype(np.complex64).astype(np.complex128)
 
 
-def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs", waves=0):
+def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs", waves=0, degrid_waves=0):
     geom = D.DeviceGeom.from_chunks(vc, N)
     dg = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
-    plan = D.TiledPlan(geom, tile=tile, chunk=chunk, layout=layout, waves=waves)
+    plan = D.TiledPlan(geom, tile=tile, chunk=chunk, layout=layout, waves=waves,
+                       degrid_waves=degrid_waves)
     plan.bin(dg)
     vis = torch.from_numpy(vc.vis.reshape(-1).astype(np.complex64)).cuda()
     g = plan.grid(dg, vis)
@@ -83,6 +84,23 @@ def test_tiled_grid_waves_vs_oracle(hip_device, tile, waves):
     assert rel_l2(g, ref) <= TOL
 
 
+@pytest.mark.parametrize("tile", [16, 32, 64])
+@pytest.mark.parametrize("degrid_waves", [2, 4, 8])
+def test_tiled_degrid_waves_vs_oracle(hip_device, tile, degrid_waves):
+    """Every choice of wavefronts per degridding workgroup (sharing one
+    staged tile), items of 7 pairs split over them."""
+    vc, gcf, _ = synth.random_small(5, grid_size=160, chunks=30, ts_ch=20, half=False,
+                                    chunk_supp="random", supports=(6, 17, 40))
+    N = 160
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N, nthreads=4)
+    ref32 = ref.astype(np.complex64).astype(np.complex128)
+    vref, _ = O.degrid(vc, gcf, ref32, nthreads=4)
+    g, v, st = run_tiled(vc, gcf, N, tile, 7, grid_in=ref32, degrid_waves=degrid_waves)
+    assert st["err"] == 0 and st["taps"] == taps
+    assert rel_l2(v, vref) <= TOL
+
+
 def test_tiled_grid_waves_rejected(hip_device):
     vc, gcf, N = synth.random_small(3)
     geom = D.DeviceGeom.from_chunks(vc, N)
@@ -90,6 +108,8 @@ def test_tiled_grid_waves_rejected(hip_device):
     from rc_amd import _lib
     with pytest.raises(_lib.RcgError, match="waves"):
         plan.set_grid_waves(5)
+    with pytest.raises(_lib.RcgError, match="waves"):
+        plan.set_degrid_waves(3)
     plan.close()
     halo = D.TiledPlan(geom, tile=32, chunk=64, layout="halo")
     with pytest.raises(_lib.RcgError, match="halo"):
