@@ -81,7 +81,8 @@ def log(*a):
 def profiled(kernel: str) -> dict:
     """Counter figures per dispatch of `kernel` at the default headline
     workload, from the rocprofv3 summary committed under profiles/ (lines
-    '<key> <kernel> <value>': traffic_bytes = L2-to-memory bytes from the
+    '<key> <kernel> <value>', the kernel named without spaces, e.g.
+    k_grid_tiles<32,3>: traffic_bytes = L2-to-memory bytes from the
     per-request-size counters, valu_instr, salu_instr)."""
     p = ROOT / "profiles" / "wproj_tiled_traffic.md"
     out = {}
@@ -140,7 +141,8 @@ def bench_batched(args) -> int:
     lognormal point counts, median 400k, clipped to 20k..5M): tile, chunk and
     gridding waves per workgroup from the autotuner, then bin + grid + degrid of all instances as one
     captured graph (one launch per kernel over all instances) against one
-    plan and one launch sequence per instance."""
+    plan and one launch sequence per instance, at the faster of the batch's
+    wavefronts per workgroup and the plan defaults."""
     import torch
     from rc_amd import autotune
     from rc_amd import device as D
@@ -181,19 +183,32 @@ def bench_batched(args) -> int:
     if st["err"]:
         print(json.dumps({"error": f"binning error word {st['err']}"}))
         return 1
-    # one plan and one launch sequence per instance, same tile and chunk
+    # one plan and one launch sequence per instance, same tile and chunk.  The
+    # wavefronts per workgroup are timed both as tuned for the batch and at the
+    # plan's defaults (more wavefronts share each work item, which suits the
+    # small instances); the faster is the baseline.
     offs = np.cumsum([0] + [g.n for g in geoms])
-    plans = [D.TiledPlan(g, tile=tile, chunk=chunk, max_support=48, waves=waves,
-                         degrid_waves=dwaves) for g in geoms]
 
-    def per_instance():
-        for k, p in enumerate(plans):
-            v, o = vis[offs[k]:offs[k + 1]], out[offs[k]:offs[k + 1]]
-            p.bin(dg)
-            p.grid(dg, v, grid[k])
-            p.degrid(dg, grid[k], o)
+    def per_instance(plans):
+        def run():
+            for k, p in enumerate(plans):
+                v, o = vis[offs[k]:offs[k + 1]], out[offs[k]:offs[k + 1]]
+                p.bin(dg)
+                p.grid(dg, v, grid[k])
+                p.degrid(dg, grid[k], o)
+        return run
 
-    ms_single = timed(per_instance, max(2, args.steps // 2), 1)
+    single, cands = {}, {}
+    for label, w, dw in (("batch_tuned", waves, dwaves), ("plan_default", 0, 0)):
+        cands[label] = [D.TiledPlan(g, tile=tile, chunk=chunk, max_support=48, waves=w,
+                                    degrid_waves=dw) for g in geoms]
+        single[label] = timed(per_instance(cands[label]), max(2, args.steps // 2), 1)
+    single_cfg = min(single, key=single.get)
+    for label, ps in cands.items():
+        if label != single_cfg:
+            for p in ps:
+                p.close()
+    plans, ms_single = cands[single_cfg], single[single_cfg]
     graphs = [p.capture(dg, D.STAGE_BIN | D.STAGE_GRID | D.STAGE_DEGRID,
                         vis=vis[offs[k]:offs[k + 1]], grid=grid[k],
                         vis_out=out[offs[k]:offs[k + 1]]) for k, p in enumerate(plans)]
@@ -236,6 +251,8 @@ def bench_batched(args) -> int:
                    "points": n, "taps_per_direction": st["taps"], "tile": tile,
                    "chunk": chunk, "waves": waves, "degrid_waves": dwaves},
         "per_instance_ms_per_step": round(ms_single, 4),
+        "per_instance_waves": {k: round(v, 4) for k, v in single.items()},
+        "per_instance_baseline": single_cfg,
         "per_instance_graphs_ms_per_step": round(ms_single_graph, 4),
         "speedup_vs_per_instance": round(ms_single / ms_batch, 3),
         "error_vs_fp64": {"grid_rel_l2": g_err, "degrid_rel_l2": d_err},
@@ -1109,15 +1126,20 @@ def main() -> int:
     # and the tap stream (8 B per tap read, mostly served from L2 and the
     # MALL, so never an HBM figure).
     t_dom = max(k_grid, k_degrid)
-    dom = "k_grid_tiles" if k_grid >= k_degrid else "k_degrid_tiles"
-    kname = f"{dom}<{args.tile}>"
+    # the kernels are templates on tile and wavefronts per workgroup (the
+    # plan's defaults: 3 gridding wavefronts at tile 32, else 4; 4 degridding)
+    if k_grid >= k_degrid:
+        dom, nw = "k_grid_tiles", args.waves or (3 if args.tile == 32 else 4)
+    else:
+        dom, nw = "k_degrid_tiles", args.degrid_waves or 4
+    kname = f"{dom}<{args.tile}, {nw}>"
     sec = t_dom / 1e3
     slots = sec * CLOCK_GHZ * 1e9 * N_CU
     fp32_tf = 8 * taps / sec / 1e12
     b_dir = vc.n * (24 + 8) + 8 * taps + 8 * N * N
     default_cfg = (args.tile == 32 and args.chunk == 1024 and args.n_times == 306 and
                    args.layout == "pairs" and args.waves == 0 and args.degrid_waves == 0)
-    prof = profiled(kname) if default_cfg else {}
+    prof = profiled(kname.replace(" ", "")) if default_cfg else {}
     tb = prof.get("traffic_bytes")
     hbm_gbs = tb / sec / 1e9 if tb else None
     fracs = {"hbm": hbm_gbs / HBM_PEAK_GBS if tb else -1.0,
