@@ -394,9 +394,9 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 // (a VALU instruction reads at most one scalar operand), which would move
 // them, and every loop-carried copy of them, into vector registers.
 //
-// Latency: a wavefront loads the records (and visibilities) of 64 pairs at
-// once, one per lane, double-buffered one batch ahead, and broadcasts them
-// with readlane; the taps of a pair's first group of up to GRP slabs are
+// Latency: a wavefront loads the records of 64 pairs at once, one per lane,
+// two batches ahead (the gridder's visibilities one batch ahead,
+// batch_records), and broadcasts them with readlane; the taps of a pair's first group of up to GRP slabs are
 // loaded while the previous pair's LDS updates run.  Two slabs per group
 // balance the loads kept in flight against the slots a short pair leaves
 // empty (measured on the 4096^2 w-projection workload, T = 32: 3 slabs are
@@ -498,10 +498,12 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
     __syncthreads();
     const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + GW - 1) / GW : 0;
     if (npw > 0) {
-      int4 rec, rec_n;
-      float2 mv, mv_n;
-      stream_batch<true>(d, vis, ir, wave, lane, npw, 0, rec, mv, GW);
-      stream_batch<true>(d, vis, ir, wave, lane, npw, 1, rec_n, mv_n, GW);
+      // records of the current batch and the next two; visibilities of the
+      // current batch and the next
+      int4 rec = batch_records(d, ir, wave, lane, npw, 0, GW);
+      int4 rec_n = batch_records(d, ir, wave, lane, npw, 1, GW);
+      int4 rec_nn = batch_records(d, ir, wave, lane, npw, 2, GW);
+      float2 mv = vis[rec.w], mv_n = vis[rec_n.w];
       // The group after (q, sq): the same pair's next slabs, or the next
       // pair's first; past the end, the empty pair (no lane active).
       auto advance = [&](Pair &q, int &sq, int &kq, float &qx, float &qy) {
@@ -512,9 +514,15 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
           if (kq < npw) {
             const int j = kq & 63;
             if (j == 0) {
-              rec = rec_n;
-              mv = mv_n;
-              stream_batch<true>(d, vis, ir, wave, lane, npw, (kq >> 6) + 1, rec_n, mv_n, GW);
+              // the moves first (vmove), then the loads into the registers
+              // they free; rec.w is not used past this point
+              rec = make_int4(vmove(rec_n.x), vmove(rec_n.y), vmove(rec_n.z), 0);
+              mv = make_float2(vmovef(mv_n.x), vmovef(mv_n.y));
+              rec_n = make_int4(vmove(rec_nn.x), vmove(rec_nn.y), vmove(rec_nn.z),
+                                vmove(rec_nn.w));
+              __builtin_amdgcn_sched_barrier(0);
+              mv_n = vis[rec_n.w];
+              rec_nn = batch_records(d, ir, wave, lane, npw, (kq >> 6) + 2, GW);
             }
             q = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                 base);
@@ -852,10 +860,10 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
     __syncthreads();
     const int npw = ir.end - ir.begin > wave ? (ir.end - ir.begin - wave + DW - 1) / DW : 0;
     if (npw > 0) {
-      int4 rec, rec_n;
-      float2 dm, dm_n;
-      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 0, rec, dm, DW);
-      stream_batch<false>(d, nullptr, ir, wave, lane, npw, 1, rec_n, dm_n, DW);
+      // records of the current batch and the next two
+      int4 rec = batch_records(d, ir, wave, lane, npw, 0, DW);
+      int4 rec_n = batch_records(d, ir, wave, lane, npw, 1, DW);
+      int4 rec_nn = batch_records(d, ir, wave, lane, npw, 2, DW);
       int k = 0, s0 = 0;
       my_codes[lane] = out_code(rec, npts);
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
@@ -885,9 +893,14 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
             if (j == 0) {
               // the slots of this batch last held the batch before the
               // previous one, whose sums are all stored
-              rec = rec_n;
+              // the moves first (vmove), then the load into the registers
+              // they free
+              rec = make_int4(vmove(rec_n.x), vmove(rec_n.y), vmove(rec_n.z), vmove(rec_n.w));
+              rec_n = make_int4(vmove(rec_nn.x), vmove(rec_nn.y), vmove(rec_nn.z),
+                                vmove(rec_nn.w));
               my_codes[(kn & 64) + lane] = out_code(rec, npts);
-              stream_batch<false>(d, nullptr, ir, wave, lane, npw, (kn >> 6) + 1, rec_n, dm_n, DW);
+              __builtin_amdgcn_sched_barrier(0);
+              rec_nn = batch_records(d, ir, wave, lane, npw, (kn >> 6) + 2, DW);
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  0);

@@ -119,23 +119,39 @@ __device__ __forceinline__ float2 &lds_at(float2 *lds, int byte)
   return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + byte);
 }
 
-// A wavefront's pairs are ir.begin + wave + nw k (nw wavefronts in the
-// workgroup), each cut into groups of GRP
-// slabs.  Their records (and visibilities) come 64 at a time, one per lane,
-// double-buffered a batch ahead and broadcast with readlane; batch b is
-// loaded by stream_batch.
-template <bool VIS>
-__device__ __forceinline__ void stream_batch(const TiledDev &d, const float2 *__restrict__ vis,
-                                             const ItemRange &ir, int wave, int lane, int npw,
-                                             int b, int4 &rec, float2 &v, int nw)
+// A register move the compiler keeps where it is written: the batch
+// rotation moves the records (and visibilities) of the coming batches into
+// the registers of the earlier ones before it issues the next loads, so
+// those loads can land in the registers the moves free.  Left to the
+// compiler, the moves become copies at the end of the block, after the
+// loads, which then target temporaries whose copies wait for them.
+__device__ __forceinline__ int vmove(int x)
 {
-  const int k = b * 64 + lane;
-  rec = make_int4(0, 0, 0, 0);
-  v = make_float2(0.f, 0.f);
-  if (k < npw) {
-    rec = d.pairs[ir.begin + wave + nw * k];
-    if (VIS) v = vis[rec.w];
-  }
+  int r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+__device__ __forceinline__ float vmovef(float x)
+{
+  return __int_as_float(vmove(__float_as_int(x)));
+}
+
+// A wavefront's pairs are ir.begin + wave + nw k (nw wavefronts in the
+// workgroup), k < npw, each cut into groups of GRP slabs.  Their records come
+// 64 at a time, one per lane (batch b: pairs 64 b .. 64 b + 63), and are
+// broadcast with readlane.  The kernels load records two batches ahead (and
+// the gridder the visibilities one batch ahead, from records that arrived a
+// batch earlier), so nothing loaded at a batch boundary is used there: a
+// use would wait for the load, and a wavefront's loads complete in order, so
+// that wait would also drain the taps in flight.  Lanes past the last pair
+// load the last pair again (never decoded), which keeps the load free of
+// branches.
+__device__ __forceinline__ int4 batch_records(const TiledDev &d, const ItemRange &ir, int wave,
+                                              int lane, int npw, int b, int nw)
+{
+  const int k = min(b * 64 + lane, npw - 1);
+  return d.pairs[ir.begin + wave + nw * k];
 }
 
 // Error of the rounded sum fl(a + b) (Knuth's TwoSum): fl(a + b) + err =
