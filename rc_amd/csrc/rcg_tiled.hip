@@ -422,7 +422,7 @@ constexpr int MAX_GRID_WAVES = 4;
 struct Pair {
   int h, rows, nslabs, point, whole, conj, slot;
   int lr;          // lane row (idle lanes: 64)
-  unsigned tb0;    // tap byte offset of the lane's slab-0 cell
+  unsigned tb0;    // tap byte offset of the lane's slab-0 cell (idle lanes: unused)
   int lb0;         // LDS byte address of the lane's slab-0 cell
   int tstep;       // tap byte step per slab (uniform)
   int lstep;       // LDS byte step per slab (uniform)
@@ -450,7 +450,9 @@ __device__ __forceinline__ Pair decode_pair(int px, int py, int pz, int pw, int 
   const int lc = lane - __mul24(lr, w);
   const bool idle = lr >= p.rows;
   p.lr = idle ? 64 : lr;
-  p.tb0 = idle ? TAP_OOB : (unsigned)(py + __mul24(lr, stride) + lc) * 8u;
+  // idle lanes (lr = 64) fail every slab's row test in load_group, which
+  // then sends their loads out of range, so tb0 needs no select here
+  p.tb0 = (unsigned)(py + __mul24(lr, stride) + lc) * 8u;
   p.lb0 = base + (__mul24(r0 + lr, LS) + c0 + lc) * 8;
   p.tstep = p.rows * stride * 8;
   p.lstep = p.rows * LS * 8;
@@ -867,7 +869,10 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
       int k = 0, s0 = 0;
       my_codes[lane] = out_code(rec, npts);
       Pair p = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane, 0);
-      float sg = p.conj ? -1.f : 1.f;
+      // +-1 from the conjugation bit by integer operations on the scalar pair
+      // field (a select of two float constants becomes a vector instruction)
+      auto sign_of = [](int conj) { return __int_as_float(0x3f800000 | (conj << 31)); };
+      float sg = sign_of(p.conj);
       float2 ta[GRP], tb[GRP];
       int la[GRP], lb[GRP];
       load_group(ta, la, p, 0, DUMMY * 8, rsrc);
@@ -904,7 +909,7 @@ k_degrid_tiles(TiledDev d, const float2 *__restrict__ grid,
             }
             pn = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                  0);
-            sgn = pn.conj ? -1.f : 1.f;
+            sgn = sign_of(pn.conj);
           } else {
             // past the end: the empty pair, no lane active; it never ends,
             // so no sum is written for it
