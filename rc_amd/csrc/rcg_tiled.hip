@@ -493,8 +493,12 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
   const __amdgpu_buffer_rsrc_t rsrc = tap_rsrc(taps, n_taps);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // LDS byte addresses of the wavefront's copy and of its dummy cell
-  const int base = wave * CP * 8, dummy = (wave * CP + DUMMY) * 8;
+  // LDS byte addresses of the wavefront's copy and of its dummy cell; the
+  // dummy address lives in a vector register (vmove) so that the address
+  // selects, whose condition is a scalar mask, need no copy of it per group
+  // (measured: gridder kernel 2.83 -> 2.80 ms on the 4096^2 w-projection
+  // workload)
+  const int base = wave * CP * 8, dummy = vmove((wave * CP + DUMMY) * 8);
   for_each_item(d, d.queue, [&](const ItemRange &ir) {
     for (int q = threadIdx.x; q < GW * CP; q += 64 * GW) acc[q] = make_float2(0.f, 0.f);
     __syncthreads();
