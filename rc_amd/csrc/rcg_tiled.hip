@@ -510,6 +510,11 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       int4 rec_n = batch_records(d, ir, wave, lane, npw, 1, GW);
       int4 rec_nn = batch_records(d, ir, wave, lane, npw, 2, GW);
       float2 mv = vis[rec.w], mv_n = vis[rec_n.w];
+      // Taps are conjugated for mirrored (w < 0) layers and, in the adjoint,
+      // once more for all of them: a pair's conj field becomes, once per
+      // pair, the sign mask of the FMA coefficients b and c (0 when the two
+      // conjugations cancel).
+      auto flip_of = [&](int conj) { return conj != d.adj ? 0 : (int)0x80000000; };
       // The group after (q, sq): the same pair's next slabs, or the next
       // pair's first; past the end, the empty pair (no lane active).
       auto advance = [&](Pair &q, int &sq, int &kq, float &qx, float &qy) {
@@ -532,6 +537,7 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
             }
             q = decode_pair<LS>(rl(rec.x, j), rl(rec.y, j), rl(rec.z, j), rl(rec.w, j), lane,
                                 base);
+            q.conj = flip_of(q.conj);
             qx = rlf(mv.x, j);
             qy = rlf(mv.y, j);
           } else {
@@ -545,6 +551,7 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
       // next group, whose successor this group issues.
       Pair pn = decode_pair<LS>(rl(rec.x, 0), rl(rec.y, 0), rl(rec.z, 0), rl(rec.w, 0), lane,
                                 base);
+      pn.conj = flip_of(pn.conj);
       int sn = 0, kn = 0;
       float vxn = rlf(mv.x, 0), vyn = rlf(mv.y, 0);
       float2 ta[GRP], tb[GRP], tc3[GRP];
@@ -574,10 +581,9 @@ k_grid_tiles(TiledDev d, const float2 *__restrict__ vis,
         advance(pnn, snn, knn, vxnn, vynn);
         load_group(t2, l2, pnn, snn, dummy, rsrc);
         __builtin_amdgcn_sched_barrier(0);
-        // taps are conjugated for mirrored (w < 0) layers and, in the
-        // adjoint, once more for all of them: b = cj ? vy : -vy and
-        // c = cj ? -vx : vx, as sign-bit flips (scalar integer ops)
-        const unsigned flip = cj != d.adj ? 0u : 0x80000000u;
+        // b = cj ? vy : -vy and c = cj ? -vx : vx as sign-bit flips
+        // (scalar integer ops) with cj's mask from flip_of
+        const unsigned flip = (unsigned)cj;
         const float a = vx, e = vy;
         const float b = __int_as_float(__float_as_int(vy) ^ flip);
         const float c = __int_as_float(__float_as_int(vx) ^ flip ^ 0x80000000u);
