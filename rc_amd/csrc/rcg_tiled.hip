@@ -325,13 +325,17 @@ k_scan_apply(TiledDev d)
   }
 }
 
-// Fill: LDS ranks within the workgroup, one global atomic per touched tile
-// for the workgroup's base, then every record is written to its slot; with
-// global bins, one global atomic per record gives its slot.
+// Fill: LDS counts per tile within the workgroup, one global atomic per
+// touched tile for the workgroup's range of slots, then every record takes
+// the next slot of its tile's range with an LDS atomic (records of a tile
+// land in any order, as with global atomics); with global bins, one global
+// atomic per record gives its slot.  Only the histogram lives in LDS, so two
+// workgroups share a CU (a per-record rank array, 73 KB at 4 points per
+// thread and 9 records per point, had left room for one).
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
 {
-  extern __shared__ int lds[];
+  extern __shared__ int hist[];  // tpi
   const BinBlock bb = bin_block(d);
   const long long t0 = (long long)bb.inst * d.tpi;
   auto put = [&](long long pos, const int4 &q) {
@@ -349,18 +353,13 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
     }
     return;
   }
-  int *hist = lds;                                           // tpi
-  uint16_t *rank = reinterpret_cast<uint16_t *>(lds + d.tpi);  // per record
   for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) hist[t] = 0;
   __syncthreads();
   for (int k = 0; k < d.ppt; k++) {
     const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
     if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
-    uint16_t *rk = rank + (k * BIN_THREADS + threadIdx.x) * d.mpp;
-    point_records(r, i, d, [&](int t, int j, const int4 &) {
-      rk[j] = (uint16_t)atomicAdd(&hist[t], 1);
-    });
+    point_records(r, i, d, [&](int t, int, const int4 &) { atomicAdd(&hist[t], 1); });
   }
   __syncthreads();
   for (int t = threadIdx.x; t < d.tpi; t += BIN_THREADS) {
@@ -372,10 +371,7 @@ k_bin_fill(const rcg_pg *__restrict__ pg, TiledDev d, int *__restrict__ err)
     const int i = bb.p0 + k * BIN_THREADS + (int)threadIdx.x;
     if (i >= bb.p1) break;
     const rcg_pg r = pg[i];
-    const uint16_t *rk = rank + (k * BIN_THREADS + threadIdx.x) * d.mpp;
-    point_records(r, i, d, [&](int t, int j, const int4 &q) {
-      put((long long)hist[t] + rk[j], q);
-    });
+    point_records(r, i, d, [&](int t, int, const int4 &q) { put(atomicAdd(&hist[t], 1), q); });
   }
 }
 
@@ -1207,19 +1203,11 @@ extern "C" rcg_status rcg_tiled_plan_create_ex(const rcg_geom *geoms, int n_inst
               "split the batch", n, max_pairs_pt);
     return RCG_ERR_ARG;
   }
-  // Binning: LDS histograms per workgroup when the instance's tiles and the
-  // fill pass's ranks fit in LDS (points per thread as many as fit), global
-  // atomics per record otherwise.
-  int ppt = 4;
-  bool gbins = tpi > MAX_BIN_TILES;
-  if (!gbins) {
-    while (ppt > 0 && tpi * 4 + (long long)BIN_THREADS * ppt * max_pairs_pt * 2 > 160 * 1024)
-      ppt >>= 1;
-    if (ppt == 0) {
-      gbins = true;
-      ppt = 4;
-    }
-  }
+  // Binning: LDS histograms per workgroup when the instance's tiles fit in
+  // LDS (MAX_BIN_TILES), global atomics per record otherwise; 4 points per
+  // thread.
+  const int ppt = 4;
+  const bool gbins = tpi > MAX_BIN_TILES;
   int nblk = 0;
   for (int k = 0; k < n_inst; k++) {
     blk_off[k] = nblk;
@@ -1412,15 +1400,13 @@ rcg_status bin_passes(rcg_tiled_plan *pl, hipStream_t st)
 {
   TiledDev &d = pl->dev;
   const size_t lds_count = d.gbins ? 0 : (size_t)d.tpi * sizeof(int);
-  const size_t lds_fill =
-      d.gbins ? 0 : lds_count + (size_t)BIN_THREADS * d.ppt * d.mpp * sizeof(uint16_t);
   const int n_seg = ceil_div(d.n_tiles, SCAN_SEG);
   hipLaunchKernelGGL(k_bin_count, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_count, st,
                      pl->pg, d, pl->err);
   hipLaunchKernelGGL(k_scan_reduce, dim3(n_seg), dim3(SCAN_SEG), 0, st, d);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_SEG), 0, st, d, n_seg);
   hipLaunchKernelGGL(k_scan_apply, dim3(n_seg), dim3(SCAN_SEG), 0, st, d);
-  hipLaunchKernelGGL(k_bin_fill, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_fill, st,
+  hipLaunchKernelGGL(k_bin_fill, dim3(pl->bin_blocks), dim3(BIN_THREADS), lds_count, st,
                      pl->pg, d, pl->err);
   RCG_HIP_TRY(hipGetLastError());
   if (d.layout == LAYOUT_HALO) {
