@@ -20,7 +20,8 @@ __device__ __forceinline__ void pregrid_record(const rcg_geom &g, const rcg_gcf 
                                                rcg_pg *__restrict__ out,
                                                unsigned long long &taps, int &err)
 {
-  const int S = g.chunk_supps[(unsigned)i / (unsigned)g.ts_ch];
+  // one point per chunk (a support per point) needs no division
+  const int S = g.chunk_supps[g.ts_ch == 1 ? (unsigned)i : (unsigned)i / (unsigned)g.ts_ch];
   PregridPoint p = pregrid_point(g.uvw[i], g.scale, g.wstep, S, g.grid_size, t.half != 0);
   rcg_pg rec;
   rec.cell = -1;
@@ -97,8 +98,17 @@ k_pregrid_batch(const rcg_geom *__restrict__ geoms, const int *__restrict__ pt_o
       if (pt_off[mid] <= i) lo = mid;
       else hi = mid - 1;
     }
-    const rcg_geom g = geoms[lo];
-    pregrid_record(g, t, i - pt_off[lo], pg + i, my_taps, my_err);
+    // The points of a wavefront nearly always belong to one instance; its
+    // geometry then comes through scalar loads rather than a vector load of
+    // the same record in every lane.
+    const int lo0 = __builtin_amdgcn_readfirstlane(lo);
+    if (__all(lo == lo0)) {
+      const rcg_geom g = geoms[lo0];
+      pregrid_record(g, t, i - pt_off[lo0], pg + i, my_taps, my_err);
+    } else {
+      const rcg_geom g = geoms[lo];
+      pregrid_record(g, t, i - pt_off[lo], pg + i, my_taps, my_err);
+    }
   }
   pregrid_flush(my_taps, my_err, taps_dev, err_dev);
 }
