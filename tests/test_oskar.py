@@ -4,8 +4,9 @@ OskarBinReader.cpp, compiled from its sources with its vendored oskar_binary
 library) returned for the same files (tests/golden/oskar_expected.npz,
 written by tests/golden/make_oskar.py): header fields equal, the CRC-32C
 failure of a changed header byte, the refusal of single-precision
-amplitudes, a missing file; and the CRC-32C the reader applies, checked on
-the standard test string."""
+amplitudes, a missing file, truncated and corrupted files (a status, never a
+crash); and the CRC-32C the reader applies, checked on the standard test
+string."""
 import ctypes as C
 
 import numpy as np
@@ -79,3 +80,28 @@ def test_crc32c_of_the_fixture_chunks():
         pos += 20 + size
         n += 1
     assert pos == len(raw) and n > 20 and ext == 1
+
+
+def test_truncated_and_corrupted_files(tmp_path):
+    """Every prefix of a fixture shorter than its first chunk, prefixes cut
+    through the later chunks, and copies with random bytes changed open with
+    an OSKAR status or succeed; none crashes or reports success for a file
+    cut inside the 64-byte header."""
+    rng = np.random.default_rng(7)
+    data = (GOLDEN / "oskar_blocks.vis").read_bytes()
+    p = tmp_path / "f.vis"
+    cases = [data[:k] for k in range(0, 400)] + [data[:k] for k in range(400, len(data), 211)]
+    for _ in range(300):
+        b = bytearray(data)
+        for i in rng.integers(0, len(b), rng.integers(1, 7)):
+            b[i] = int(rng.integers(0, 256))
+        cases.append(bytes(b))
+    for k, c in enumerate(cases):
+        p.write_bytes(c)
+        try:
+            vd, _ = oskar.open_header(p)
+        except oskar.OskarError as e:
+            assert e.status != 0
+            continue
+        oskar._fns().freeBinHandler(C.byref(vd))
+        assert len(c) >= 64, k
