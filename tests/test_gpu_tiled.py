@@ -165,3 +165,32 @@ def test_compensated_flush_vs_oracle(hip_device):
     assert e_comp <= e_plain
     assert rel_l2(again, ref) <= TOL
     plan.close()
+
+
+@pytest.mark.parametrize("n,chunk", [(192, 4096), (195, 4096), (387, 4096), (771, 4096),
+                                     (771, 192)])
+def test_tiled_record_batch_boundaries(hip_device, n, chunk):
+    """Every window inside one 32x32 tile (cells 64..95 of a 160^2 grid), so
+    the tile holds exactly n pairs; with 3 wavefronts per gridding and 4 per
+    degridding workgroup a wavefront walks 64, 65, 129 or 257 pairs (gridder)
+    and 48..193 (degridder): record batches of 64 that end exactly at, just
+    before and past the item's last pair, and items of 192 pairs."""
+    vc, gcf, _ = synth.random_small(11, grid_size=160, chunks=n, ts_ch=1, half=True,
+                                    supports=(6, 8, 10), oob_frac=0.0)
+    rng = np.random.default_rng(n)
+    # u, v in [-10, 10] pixels (scale 1): a window's first cell is 80 - S/2 + floor(u),
+    # 65..86 for S in {6, 8, 10}, so every S x S window lies in cells 64..95
+    uv = rng.uniform(-10.0, 10.0, (n, 2))
+    vc.uvw[:, 0, 0] = uv[:, 0] / vc.scale
+    vc.uvw[:, 0, 1] = uv[:, 1] / vc.scale
+    N = 160
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N, nthreads=4)
+    ref32 = ref.astype(np.complex64).astype(np.complex128)
+    vref, _ = O.degrid(vc, gcf, ref32, nthreads=4)
+    g, v, st = run_tiled(vc, gcf, N, 32, chunk, grid_in=ref32, waves=3, degrid_waves=4)
+    assert st["err"] == 0
+    assert st["taps"] == taps
+    assert st["pairs"] == n
+    assert rel_l2(g, ref) <= TOL
+    assert rel_l2(v, vref) <= TOL
