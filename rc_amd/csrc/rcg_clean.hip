@@ -20,6 +20,14 @@
 // another and no device-wide fence is needed.  Peak, PSF peak, a stop flag
 // and the iteration count live in device memory, so no host
 // synchronisation is needed between iterations.
+//
+// Measured, 1000 FP32 iterations on a 2048^2 dirty image: 17.8 ms with
+// 1024-thread blocks whose threads issue all 8 of their pixels' loads
+// together and keep a running maximum (24.4 ms with 256 threads and 4 loads
+// in flight, 18.4 ms without the running maximum).  Not taken: one
+// cooperative launch for the whole cycle, the residual held in registers and
+// a grid barrier per iteration, 27.0 ms; the barrier across the 8 XCDs
+// costs more than a kernel boundary.
 #include <cfloat>
 
 #include "rcg_internal.h"
@@ -28,7 +36,8 @@ namespace rcg {
 namespace {
 
 constexpr int CB = 512;   // blocks of the partial-maximum pass
-constexpr int CT = 256;   // threads per block
+constexpr int CT = 1024;  // threads per block (at most; FP64 passes use 512)
+constexpr int CU = 8;     // pixels of a thread whose loads are issued together
 
 struct CleanState {
   long long pos;    // residual peak (row*pitch + col)
@@ -88,7 +97,7 @@ __device__ __forceinline__ Best block_best(Best b, Best *sh)
 __device__ __forceinline__ Best reduce_partials(const Partials *__restrict__ p, Best *sh)
 {
   Best b{-1.0, 0x7fffffffffffffffLL, 0.0};
-  for (int k = threadIdx.x; k < CB; k += CT) b = better(b, Best{p->a[k], p->idx[k], p->v[k]});
+  for (int k = threadIdx.x; k < CB; k += blockDim.x) b = better(b, Best{p->a[k], p->idx[k], p->v[k]});
   return block_best(b, sh);
 }
 
@@ -119,13 +128,15 @@ k_clean_pass(T *__restrict__ img, const T *__restrict__ psf, T *__restrict__ mod
              int find, const Partials *__restrict__ pin, Partials *__restrict__ pout)
 {
 #pragma clang fp contract(off)
-  __shared__ Best sh[CT / 64];
+  // separate slots for the two block reductions, so that the second one's
+  // writes cannot meet a slow wavefront's read of the first one's result
+  __shared__ Best sh_peak[CT / 64], sh_out[CT / 64];
   long long dr = 0, dc = 0;
   T a = 0;
   const bool subtract = it > 0;
   if (subtract) {
     if (st->done) return;
-    const Best pk = reduce_partials(pin, sh);
+    const Best pk = reduce_partials(pin, sh_peak);
     const long long r = pk.idx / siz, c = pk.idx % siz;
     const long long pos = r * pitch + c;
     const double pval = st->pval;
@@ -149,29 +160,63 @@ k_clean_pass(T *__restrict__ img, const T *__restrict__ psf, T *__restrict__ mod
     dc = pc - c;
     a = (T)g;
   }
-  Best b{-1.0, 0x7fffffffffffffffLL, 0.0};
-  for (int r = blockIdx.x; r < siz; r += gridDim.x) {
-    T *row = img + (long long)r * pitch;
-    const long long sr = r + dr;
-    const bool in_r = sr >= 0 && sr < siz;
-    // unrolled so that a thread's loads of several columns are in flight
-    // together (img and psf do not alias)
-#pragma unroll 4
-    for (int c = threadIdx.x; c < siz; c += CT) {
-      T x = row[c];
-      if (subtract && in_r) {
-        const long long sc = c + dc;
-        if (sc >= 0 && sc < siz) {
-          x -= a * psf[sr * pitch + sc];
-          row[c] = x;
-        }
+  // A thread's pixels are rows blockIdx.x + k * gridDim.x, columns
+  // threadIdx.x + j * blockDim.x, taken CU at a time in (k, j) order.  Every
+  // load of a batch (image, and PSF where the shifted PSF covers the pixel)
+  // is issued before any is used, so a batch costs one memory latency: at
+  // 2048^2 a thread's 8 pixels are one batch.  The pixels of a batch are
+  // distinct and img and psf do not alias, so the stores may follow all the
+  // loads.  The thread's running maximum: its pixels come in increasing index
+  // order, so a strict > keeps the first index on ties, as better() does.
+  T ba = -1, bv = 0;
+  int br = -1, bc = 0;
+  const int tx = threadIdx.x, bx = blockIdx.x, nb = gridDim.x, nt = blockDim.x;
+  const int nj = tx < siz ? (siz - 1 - tx) / nt + 1 : 0;
+  const int nk = bx < siz ? (siz - 1 - bx) / nb + 1 : 0;
+  const int ne = nj * nk;
+  int k = 0, j = 0;
+  for (int e0 = 0; e0 < ne; e0 += CU) {
+    T x[CU], p[CU];
+    int rr[CU], cc[CU];
+    bool sb[CU];
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const int r = bx + k * nb, c = tx + j * nt;
+      const bool ok = e0 + u < ne;
+      const long long sr = r + dr, sc = c + dc;
+      rr[u] = r;
+      cc[u] = c;
+      sb[u] = ok && subtract && sr >= 0 && sr < siz && sc >= 0 && sc < siz;
+      x[u] = ok ? img[(long long)r * pitch + c] : T(0);
+      p[u] = sb[u] ? psf[sr * pitch + sc] : T(0);
+      if (++j == nj) {
+        j = 0;
+        k++;
       }
-      if (find) b = better(b, Best{(double)fabs(x), (long long)r * siz + c, (double)x});
+    }
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      if (sb[u]) {
+        x[u] -= a * p[u];
+        img[(long long)rr[u] * pitch + cc[u]] = x[u];
+      }
+      if (find && e0 + u < ne && fabs(x[u]) > ba) {
+        ba = fabs(x[u]);
+        bv = x[u];
+        br = rr[u];
+        bc = cc[u];
+      }
     }
   }
   if (!find) return;
-  b = block_best(b, sh);
-  if (threadIdx.x == 0) {
+  Best b{-1.0, 0x7fffffffffffffffLL, 0.0};
+  if (br >= 0) b = Best{(double)ba, (long long)br * siz + bc, (double)bv};
+  // block maximum to thread 0 only (one barrier)
+  b = wave_best(b);
+  if ((tx & 63) == 0) sh_out[tx >> 6] = b;
+  __syncthreads();
+  if (tx == 0) {
+    for (int w = 1; w < (nt >> 6); w++) b = better(b, sh_out[w]);
     pout->a[blockIdx.x] = b.a;
     pout->idx[blockIdx.x] = b.idx;
     pout->v[blockIdx.x] = b.v;
@@ -193,9 +238,12 @@ rcg_status hogbom(T *model, T *res, const T *psf, int siz, int pitch, unsigned n
   CleanState *cs = static_cast<CleanState *>(scratch);
   Partials *pp = reinterpret_cast<Partials *>(cs + 1);  // two buffers, passes alternate
   const long long n = (long long)siz * pitch;
+  // FP64 passes need more registers per thread (83 against 59 VGPRs at CU = 8):
+  // 512 threads keep two blocks per CU resident
+  const int nt = sizeof(T) == 8 ? 512 : CT;
   hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(256), 0, st, model, n);
   // PSF peak
-  hipLaunchKernelGGL(k_clean_pass<T>, dim3(CB), dim3(CT), 0, st, const_cast<T *>(psf), psf,
+  hipLaunchKernelGGL(k_clean_pass<T>, dim3(CB), dim3(nt), 0, st, const_cast<T *>(psf), psf,
                      model, siz, pitch, cs, gain, threshold, 0, 1, pp + 1, pp);
   hipLaunchKernelGGL(k_clean_psf_peak, dim3(1), dim3(CT), 0, st, cs, siz, pitch, pp);
   if (niters == 0) {
@@ -205,7 +253,7 @@ rcg_status hogbom(T *model, T *res, const T *psf, int siz, int pitch, unsigned n
   // pass 0: the maxima of the residual; pass it (1..niters): peak it - 1,
   // its subtraction and, but for the last pass, the next maxima
   for (unsigned it = 0; it <= niters; it++)
-    hipLaunchKernelGGL(k_clean_pass<T>, dim3(CB), dim3(CT), 0, st, res, psf, model, siz, pitch,
+    hipLaunchKernelGGL(k_clean_pass<T>, dim3(CB), dim3(nt), 0, st, res, psf, model, siz, pitch,
                        cs, gain, threshold, (int)it, it < niters ? 1 : 0, pp + ((it + 1) & 1),
                        pp + (it & 1));
   RCG_HIP_TRY(hipGetLastError());

@@ -143,6 +143,44 @@ def test_hogbom_device_fp32_iterations_and_threshold(hip_device):
     assert np.abs(res.cpu().numpy()).max() < 1e-2
 
 
+@pytest.mark.parametrize("f64", [True, False])
+@pytest.mark.parametrize("n,p", [(2000, 2008), (2600, 2608)])
+def test_hogbom_ragged_large_vs_oracle(hip_device, f64, n, p):
+    """Ragged sizes in a padded pitch, with a PSF whose sidelobes cover the
+    whole image, against the oracle's Hogbom (FP64 to 1e-12, FP32 to 1e-5):
+    several rows per block and columns per thread, and at 2600^2 more than
+    one batch of loads per thread."""
+    from rc_amd import device as D
+    yy, xx = np.mgrid[0:n, 0:n]
+    c = n // 2
+    psf = np.zeros((n, p))
+    psf[:, :n] = (np.exp(-((yy - c) ** 2 + (xx - c) ** 2) / 4.5)
+                  + 0.02 * np.cos(0.05 * (yy - c)) * np.cos(0.07 * (xx - c)))
+    dirty = np.zeros((n, p))
+    srcs = [(700, n - 500, 2.0), (n - 700, 300, -1.3)] if f64 else [(700, n - 500, 2.0)]
+    for (r, cc, f) in srcs:
+        i0, i1 = max(0, r - c), min(n, n + r - c)
+        j0, j1 = max(0, cc - c), min(n, n + cc - c)
+        dirty[i0:i1, j0:j1] += f * psf[i0 - r + c:i1 - r + c, j0 - cc + c:j1 - cc + c]
+    dirty[:, :n] += 1e-3 * np.random.default_rng(5).standard_normal((n, n))
+    niters = 25
+    res_ref = dirty.copy()
+    mod_ref = np.zeros_like(dirty)
+    it_ref = O.oracle_lib().orc_hogbom(mod_ref.ctypes.data, res_ref.ctypes.data,
+                                       psf.ctypes.data, n, p, niters, 0.1, 0.0)
+    dt = torch.float64 if f64 else torch.float32
+    res = torch.from_numpy(dirty).to(dt).cuda()
+    ps = torch.from_numpy(psf).to(dt).cuda()
+    mod, it = D.hogbom(res, ps, niters, 0.1, 0.0)
+    assert it == it_ref == niters
+    tol = 1e-12 if f64 else 1e-5
+    m = mod.cpu().numpy().astype(np.float64)
+    r = res.cpu().numpy().astype(np.float64)
+    assert np.array_equal(m != 0, mod_ref != 0)
+    assert rel_l2(m, mod_ref) <= tol
+    assert rel_l2(r[:, :n], res_ref[:, :n]) <= tol
+
+
 def test_vis5_unpack_rotate_and_pack(hip_device):
     """MS6 five-double records -> uvw + visibilities (rotated by
     exp(2 pi i w scale) in FP64 against the oracle's rotateCPU) and back."""
