@@ -37,7 +37,7 @@ namespace {
 
 constexpr int CB = 512;   // blocks of the partial-maximum pass
 constexpr int CT = 1024;  // threads per block (at most; FP64 passes use 512)
-constexpr int CU = 8;     // pixels of a thread whose loads are issued together
+constexpr int PB = 8;     // pixels of a thread whose loads are issued together (a batch)
 
 struct CleanState {
   long long pos;    // residual peak (row*pitch + col)
@@ -161,7 +161,7 @@ k_clean_pass(T *__restrict__ img, const T *__restrict__ psf, T *__restrict__ mod
     a = (T)g;
   }
   // A thread's pixels are rows blockIdx.x + k * gridDim.x, columns
-  // threadIdx.x + j * blockDim.x, taken CU at a time in (k, j) order.  Every
+  // threadIdx.x + j * blockDim.x, taken PB at a time in (k, j) order.  Every
   // load of a batch (image, and PSF where the shifted PSF covers the pixel)
   // is issued before any is used, so a batch costs one memory latency: at
   // 2048^2 a thread's 8 pixels are one batch.  The pixels of a batch are
@@ -175,12 +175,12 @@ k_clean_pass(T *__restrict__ img, const T *__restrict__ psf, T *__restrict__ mod
   const int nk = bx < siz ? (siz - 1 - bx) / nb + 1 : 0;
   const int ne = nj * nk;
   int k = 0, j = 0;
-  for (int e0 = 0; e0 < ne; e0 += CU) {
-    T x[CU], p[CU];
-    int rr[CU], cc[CU];
-    bool sb[CU];
+  for (int e0 = 0; e0 < ne; e0 += PB) {
+    T x[PB], p[PB];
+    int rr[PB], cc[PB];
+    bool sb[PB];
 #pragma unroll
-    for (int u = 0; u < CU; u++) {
+    for (int u = 0; u < PB; u++) {
       const int r = bx + k * nb, c = tx + j * nt;
       const bool ok = e0 + u < ne;
       const long long sr = r + dr, sc = c + dc;
@@ -195,7 +195,7 @@ k_clean_pass(T *__restrict__ img, const T *__restrict__ psf, T *__restrict__ mod
       }
     }
 #pragma unroll
-    for (int u = 0; u < CU; u++) {
+    for (int u = 0; u < PB; u++) {
       if (sb[u]) {
         x[u] -= a * p[u];
         img[(long long)rr[u] * pitch + cc[u]] = x[u];
@@ -238,7 +238,7 @@ rcg_status hogbom(T *model, T *res, const T *psf, int siz, int pitch, unsigned n
   CleanState *cs = static_cast<CleanState *>(scratch);
   Partials *pp = reinterpret_cast<Partials *>(cs + 1);  // two buffers, passes alternate
   const long long n = (long long)siz * pitch;
-  // FP64 passes need more registers per thread (83 against 59 VGPRs at CU = 8):
+  // FP64 passes need more registers per thread (82 against 57 VGPRs at PB = 8):
   // 512 threads keep two blocks per CU resident
   const int nt = sizeof(T) == 8 ? 512 : CT;
   hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(256), 0, st, model, n);
