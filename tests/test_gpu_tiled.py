@@ -25,8 +25,9 @@ def c64_round(vc, gcf):
     return vc32, gcf.astype(np.complex64).astype(np.complex128)
 
 
-def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs", waves=0, degrid_waves=0):
-    geom = D.DeviceGeom.from_chunks(vc, N)
+def run_tiled(vc, gcf, N, tile, chunk, grid_in=None, layout="pairs", waves=0, degrid_waves=0,
+              pitch=None):
+    geom = D.DeviceGeom.from_chunks(vc, N, pitch)
     dg = D.DeviceGCF.from_table(gcf, dtype=np.complex64)
     plan = D.TiledPlan(geom, tile=tile, chunk=chunk, layout=layout, waves=waves,
                        degrid_waves=degrid_waves)
@@ -130,6 +131,26 @@ def test_tiled_wproj_slice_vs_oracle(hip_device, layout, tile):
     assert st["taps"] == taps and st["err"] == 0
     assert rel_l2(g, ref) <= TOL
     assert rel_l2(v, vref) <= TOL
+
+
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_tiled_ragged_grid_padded_pitch_vs_oracle(hip_device, tile):
+    """A 600^2 grid (not a multiple of any tile: the last row and column of
+    tiles are partial) in rows of 613 elements, windows up to the grid's
+    edges and a tenth of the points pushed outside it."""
+    vc, gcf, _ = synth.random_small(23, grid_size=600, chunks=30, ts_ch=20, max_plane=2,
+                                    half=True, supports=(6, 12, 20))
+    N, P = 600, 613
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N, P, nthreads=4)
+    ref32 = ref.astype(np.complex64).astype(np.complex128)
+    vref, _ = O.degrid(vc, gcf, ref32, nthreads=4)
+    g, v, st = run_tiled(vc, gcf, N, tile, 64, grid_in=ref32, pitch=P)
+    assert g.shape == (N, P)
+    assert st["err"] == 0 and st["taps"] == taps
+    assert rel_l2(g[:, :N], ref[:, :N]) <= TOL
+    assert rel_l2(v, vref) <= TOL
+    assert np.array_equal(v == 0, vref == 0)  # points left out are 0
 
 
 def test_tiled_support_above_plan_is_reported(hip_device):
