@@ -153,6 +153,18 @@ def test_tiled_ragged_grid_padded_pitch_vs_oracle(hip_device, tile):
     assert np.array_equal(v == 0, vref == 0)  # points left out are 0
 
 
+def test_tiled_all_points_outside(hip_device):
+    """Every point outside the grid: no pairs and no taps, a zero grid, and
+    zero visibilities degridded from a grid of ones."""
+    vc, gcf, N = synth.random_small(24, grid_size=128, chunks=8, ts_ch=5, oob_frac=1.0)
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N)
+    assert taps == 0 and not ref.any()
+    g, v, st = run_tiled(vc, gcf, N, 32, 64, grid_in=np.ones((N, N)))
+    assert st["err"] == 0 and st["taps"] == 0 and st["pairs"] == 0
+    assert not g.any() and not v.any()
+
+
 def test_tiled_support_above_plan_is_reported(hip_device):
     vc, gcf, N = synth.random_small(5, grid_size=96, supports=(6, 8, 10))
     geom = D.DeviceGeom.from_chunks(vc, N)

@@ -69,6 +69,34 @@ def test_tiled_f64_padded_pitch_large_supports(hip_device):
     assert rel_l2(v, vref) <= TOL
 
 
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_tiled_f64_ragged_grid_padded_pitch_vs_oracle(hip_device, tile):
+    """A 600^2 grid in rows of 613 elements: the last row and column of
+    tiles are partial at every tile size, windows reach the grid's edges and
+    a tenth of the points lie outside it."""
+    vc, gcf, _ = synth.random_small(23, grid_size=600, chunks=30, ts_ch=20, max_plane=2,
+                                    half=True, supports=(6, 12, 20))
+    N, P = 600, 613
+    ref, taps = O.grid(vc, gcf, N, P, nthreads=4)
+    vref, _ = O.degrid(vc, gcf, ref, nthreads=4)
+    g, st, v = run_f64(vc, gcf, N, tile, 64, P, grid_in=ref)
+    assert st["err"] == 0 and st["taps"] == taps
+    assert rel_l2(g, ref) <= TOL
+    assert rel_l2(v, vref) <= TOL
+    assert np.array_equal(v == 0, vref == 0)
+
+
+def test_tiled_f64_all_points_outside(hip_device):
+    """Every point outside the grid: no taps, a zero grid, zero
+    visibilities."""
+    vc, gcf, N = synth.random_small(24, grid_size=128, chunks=8, ts_ch=5, oob_frac=1.0)
+    ref, taps = O.grid(vc, gcf, N)
+    assert taps == 0 and not ref.any()
+    g, st, v = run_f64(vc, gcf, N, 32, 64, grid_in=ref + 1.0)
+    assert st["err"] == 0 and st["taps"] == 0 and st["pairs"] == 0
+    assert not g.any() and not v.any()
+
+
 def test_tiled_f64_wproj_slice_vs_oracle(hip_device):
     """128k points of the earth-rotation workload on the full 4096^2 grid
     with 32 w-planes (supports 8..64), complex128 throughout."""
