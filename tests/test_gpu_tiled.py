@@ -153,6 +153,25 @@ def test_tiled_ragged_grid_padded_pitch_vs_oracle(hip_device, tile):
     assert np.array_equal(v == 0, vref == 0)  # points left out are 0
 
 
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_tiled_one_pair_per_item_vs_oracle(hip_device, tile):
+    """chunk = 1: every pair is a work item of its own, so nearly every tile
+    has several items (atomic flushes of the copies' sums) and the queues
+    hand out one item per pair."""
+    vc, gcf, _ = synth.random_small(25, grid_size=160, chunks=30, ts_ch=20, half=True,
+                                    supports=(6, 17, 40))
+    N = 160
+    vc, gcf = c64_round(vc, gcf)
+    ref, taps = O.grid(vc, gcf, N, nthreads=4)
+    ref32 = ref.astype(np.complex64).astype(np.complex128)
+    vref, _ = O.degrid(vc, gcf, ref32, nthreads=4)
+    g, v, st = run_tiled(vc, gcf, N, tile, 1, grid_in=ref32)
+    assert st["err"] == 0 and st["taps"] == taps
+    assert st["items"] == st["pairs"]
+    assert rel_l2(g, ref) <= TOL
+    assert rel_l2(v, vref) <= TOL
+
+
 def test_tiled_all_points_outside(hip_device):
     """Every point outside the grid: no pairs and no taps, a zero grid, and
     zero visibilities degridded from a grid of ones."""
